@@ -1,0 +1,317 @@
+"""ctypes binding of the C-ABI in include/astroflow_amd.h.
+
+The library is loaded from the package directory (built in-tree by
+``python -m astroflow_amd.build``).  There is no fallback: a missing library
+raises, so nothing silently runs on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libastroflow_amd.so")
+
+AF_OK, AF_ERR_INVALID, AF_ERR_RUNTIME, AF_ERR_NOMEM, AF_ERR_NODEV, AF_ERR_BUSY = range(6)
+AF_U8, AF_U16, AF_U32, AF_F32 = 1, 2, 3, 4
+METHOD_DIRECT, METHOD_SUBBAND = 0, 1
+
+DTYPE_CODE = {np.dtype(np.uint8): AF_U8, np.dtype(np.uint16): AF_U16,
+              np.dtype(np.uint32): AF_U32, np.dtype(np.float32): AF_F32}
+CODE_DTYPE = {v: k for k, v in DTYPE_CODE.items()}
+OUT_DTYPE = {AF_U8: np.dtype(np.uint32), AF_U16: np.dtype(np.uint32),
+             AF_U32: np.dtype(np.uint64), AF_F32: np.dtype(np.float32)}
+
+
+class AFError(RuntimeError):
+    pass
+
+
+class PlanDesc(C.Structure):
+    _fields_ = [
+        ("dtype", C.c_int), ("nchan", C.c_int), ("nsamp", C.c_size_t), ("ds", C.c_int),
+        ("tsamp", C.c_double), ("freq_mhz", C.POINTER(C.c_double)), ("f_ref_mhz", C.c_double),
+        ("dd_c0", C.c_int), ("dd_c1", C.c_int), ("dm_low", C.c_double), ("dm_step", C.c_double),
+        ("ndm", C.c_int), ("nout", C.c_size_t), ("method", C.c_int), ("sb_nchan", C.c_int),
+        ("ndm0", C.c_int),
+        ("use_iqrm", C.c_int), ("iqrm_c0", C.c_int), ("iqrm_c1", C.c_int),
+        ("iqrm_mode", C.c_int), ("iqrm_radius_frac", C.c_float), ("iqrm_nsigma", C.c_float),
+        ("iqrm_geofactor", C.c_double), ("iqrm_win", C.c_uint), ("iqrm_hop", C.c_uint),
+        ("iqrm_include_tail", C.c_int),
+        ("use_zero_dm", C.c_int), ("zdm_c0", C.c_int), ("zdm_c1", C.c_int),
+        ("bad_chans", C.POINTER(C.c_int)), ("n_bad", C.c_int),
+        ("tile_t", C.c_int), ("dm_per_block", C.c_int),
+    ]
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [("nbin", C.c_size_t), ("nout", C.c_size_t), ("ndm", C.c_int),
+                ("out_elem_bytes", C.c_int), ("nwin", C.c_int), ("nsb", C.c_int),
+                ("ndm_nom", C.c_int), ("max_delay", C.c_int), ("scratch_bytes", C.c_size_t),
+                ("adds", C.c_double)]
+
+
+class StageTimes(C.Structure):
+    _fields_ = [("bin", C.c_float), ("stats", C.c_float), ("iqrm", C.c_float),
+                ("clean", C.c_float), ("dedisp", C.c_float), ("stage2", C.c_float),
+                ("total", C.c_float), ("n", C.c_int)]
+
+
+_lib = None
+
+# name -> (restype, argtypes); every symbol declared in include/astroflow_amd.h
+_P, _I, _S = C.c_void_p, C.c_int, C.c_size_t
+SIGNATURES = {
+    "af_plan_create": (_I, [C.POINTER(PlanDesc), C.POINTER(_P)]),
+    "af_plan_destroy": (_I, [_P]),
+    "af_plan_query": (_I, [_P, C.POINTER(PlanInfo)]),
+    "af_plan_tables": (_I, [_P, _P, _P]),
+    "af_plan_windows": (_I, [_P, _P, _P]),
+    "af_plan_set_profiling": (_I, [_P, _I]),
+    "af_plan_stage_times": (_I, [_P, C.POINTER(StageTimes)]),
+    "af_execute": (_I, [_P, _P, _P, _P, _P]),
+    "af_execute_host": (_I, [_P, _P, _P, _P]),
+    "af_device_count": (_I, [C.POINTER(_I)]),
+    "af_set_device": (_I, [_I]),
+    "af_malloc": (_I, [C.POINTER(_P), _S]),
+    "af_free": (_I, [_P]),
+    "af_memcpy_h2d": (_I, [_P, _P, _S, _P]),
+    "af_memcpy_d2h": (_I, [_P, _P, _S, _P]),
+    "af_memset": (_I, [_P, _I, _S, _P]),
+    "af_stream_create": (_I, [C.POINTER(_P)]),
+    "af_stream_destroy": (_I, [_P]),
+    "af_stream_sync": (_I, [_P]),
+    "af_sync": (_I, []),
+    "af_time_execute": (_I, [_P, _P, _P, _I, C.POINTER(C.c_float)]),
+    "af_last_error": (C.c_char_p, []),
+    "af_version": (C.c_char_p, []),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise AFError(f"{LIB_PATH} is not built; run `python -m astroflow_amd.build`")
+        _lib = C.CDLL(LIB_PATH)
+        for name, (r, a) in SIGNATURES.items():
+            f = getattr(_lib, name)
+            f.restype = r
+            f.argtypes = a
+    return _lib
+
+
+_EXC = {AF_ERR_INVALID: ValueError, AF_ERR_NOMEM: MemoryError}
+
+
+def check(rc: int, what: str = ""):
+    if rc != AF_OK:
+        msg = lib().af_last_error().decode(errors="replace")
+        exc = _EXC.get(rc, AFError)
+        raise exc(f"{what}: {msg}" if what else msg)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(lib().af_device_count(C.byref(n)), "af_device_count")
+    return n.value
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+@dataclass
+class DedispSpec:
+    """One dedispersion call, in the C-ABI's explicit terms (af_plan_desc)."""
+    dtype: np.dtype
+    nchan: int
+    nsamp: int
+    freq: np.ndarray
+    tsamp: float
+    dm_low: float
+    dm_step: float
+    ndm: int
+    dd_c0: int
+    dd_c1: int
+    f_ref: float
+    ds: int = 1
+    nout: Optional[int] = None
+    method: str = "subband"
+    sbw: int = 32
+    ndm0: int = 32
+    use_iqrm: bool = False
+    iqrm_c0: int = 0
+    iqrm_c1: int = 0
+    iqrm_mode: int = 1
+    iqrm_radius_frac: float = 0.1
+    iqrm_nsigma: float = 3.0
+    iqrm_geofactor: float = 1.5
+    iqrm_W: int = 0
+    iqrm_H: int = 0
+    iqrm_include_tail: bool = True
+    use_zero_dm: bool = False
+    zdm_c0: int = 0
+    zdm_c1: int = 0
+    bad_channels: List[int] = field(default_factory=list)
+    tile_t: int = 0
+    dm_per_block: int = 0
+
+    def to_desc(self):
+        """Returns (PlanDesc, keepalive) -- keepalive owns the arrays it points to."""
+        freq = np.ascontiguousarray(self.freq, np.float64)
+        bad = np.ascontiguousarray(self.bad_channels, np.int32)
+        d = PlanDesc()
+        d.dtype = DTYPE_CODE[np.dtype(self.dtype)]
+        d.nchan = self.nchan
+        d.nsamp = self.nsamp
+        d.ds = self.ds
+        d.tsamp = self.tsamp
+        d.freq_mhz = freq.ctypes.data_as(C.POINTER(C.c_double))
+        d.f_ref_mhz = self.f_ref
+        d.dd_c0, d.dd_c1 = self.dd_c0, self.dd_c1
+        d.dm_low, d.dm_step, d.ndm = self.dm_low, self.dm_step, self.ndm
+        d.nout = self.nout or 0
+        d.method = METHOD_DIRECT if self.method == "direct" else METHOD_SUBBAND
+        d.sb_nchan, d.ndm0 = self.sbw, self.ndm0
+        d.use_iqrm = int(self.use_iqrm)
+        d.iqrm_c0, d.iqrm_c1, d.iqrm_mode = self.iqrm_c0, self.iqrm_c1, self.iqrm_mode
+        d.iqrm_radius_frac, d.iqrm_nsigma = self.iqrm_radius_frac, self.iqrm_nsigma
+        d.iqrm_geofactor = self.iqrm_geofactor
+        d.iqrm_win, d.iqrm_hop = self.iqrm_W, self.iqrm_H
+        d.iqrm_include_tail = int(self.iqrm_include_tail)
+        d.use_zero_dm, d.zdm_c0, d.zdm_c1 = int(self.use_zero_dm), self.zdm_c0, self.zdm_c1
+        d.bad_chans = bad.ctypes.data_as(C.POINTER(C.c_int)) if len(bad) else None
+        d.n_bad = len(bad)
+        d.tile_t, d.dm_per_block = self.tile_t, self.dm_per_block
+        return d, (freq, bad)
+
+
+class Plan:
+    """Owns an af_plan.  Device buffers are raw pointers (ints)."""
+
+    def __init__(self, spec: DedispSpec):
+        self.spec = spec
+        desc, keep = spec.to_desc()
+        h = C.c_void_p()
+        check(lib().af_plan_create(C.byref(desc), C.byref(h)), "af_plan_create")
+        self._h = h
+        info = PlanInfo()
+        check(lib().af_plan_query(h, C.byref(info)), "af_plan_query")
+        self.info = info
+        self.in_dtype = np.dtype(spec.dtype)
+        self.out_dtype = OUT_DTYPE[DTYPE_CODE[self.in_dtype]]
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().af_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def out_shape(self):
+        return (self.info.ndm, self.info.nout)
+
+    @property
+    def nwin(self):
+        return self.info.nwin
+
+    def tables(self):
+        s = self.spec
+        nch = s.dd_c1 - s.dd_c0
+        rows = self.info.ndm if s.method == "direct" else self.info.ndm_nom
+        d = np.empty((rows, nch), np.int32)
+        res = None
+        if s.method != "direct":
+            res = np.empty((self.info.ndm, self.info.nsb), np.int32)
+        check(lib().af_plan_tables(self._h, _ptr(d), _ptr(res) if res is not None else None))
+        return d, res
+
+    def windows(self):
+        n = self.info.nwin
+        t0 = np.empty(max(n, 1), np.uint32)
+        t1 = np.empty(max(n, 1), np.uint32)
+        check(lib().af_plan_windows(self._h, _ptr(t0), _ptr(t1)))
+        return t0[:n].copy(), t1[:n].copy()
+
+    def run_host(self, x: np.ndarray, want_masks: bool = False):
+        x = np.ascontiguousarray(x)
+        if x.dtype != self.in_dtype:
+            raise ValueError(f"input dtype {x.dtype} != plan dtype {self.in_dtype}")
+        if x.size != self.spec.nsamp * self.spec.nchan:
+            raise ValueError("input size does not match the plan")
+        out = np.empty(self.out_shape, self.out_dtype)
+        masks = None
+        if want_masks and self.nwin:
+            masks = np.empty((self.nwin, self.spec.nchan), np.uint8)
+        check(lib().af_execute_host(self._h, _ptr(x), _ptr(out),
+                                    _ptr(masks) if masks is not None else None),
+              "af_execute_host")
+        return out, masks
+
+    def execute(self, d_in: int, d_out: int, d_masks: int = 0, stream: int = 0):
+        check(lib().af_execute(self._h, C.c_void_p(d_in), C.c_void_p(d_out),
+                               C.c_void_p(d_masks) if d_masks else None,
+                               C.c_void_p(stream) if stream else None), "af_execute")
+
+    def time_execute(self, d_in: int, d_out: int, reps: int) -> float:
+        ms = C.c_float(0)
+        check(lib().af_time_execute(self._h, C.c_void_p(d_in), C.c_void_p(d_out), reps,
+                                    C.byref(ms)), "af_time_execute")
+        return ms.value
+
+    def set_profiling(self, on: bool):
+        check(lib().af_plan_set_profiling(self._h, int(on)))
+
+    def stage_times(self) -> dict:
+        t = StageTimes()
+        check(lib().af_plan_stage_times(self._h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in StageTimes._fields_}
+
+
+class DeviceBuffer:
+    """A device allocation made through the C-ABI (no torch needed)."""
+
+    def __init__(self, nbytes: int):
+        p = C.c_void_p()
+        check(lib().af_malloc(C.byref(p), nbytes), "af_malloc")
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib().af_memcpy_h2d(C.c_void_p(self.ptr), _ptr(a), a.nbytes, None), "h2d")
+
+    def download(self, a: np.ndarray):
+        assert a.flags.c_contiguous and a.nbytes <= self.nbytes
+        check(lib().af_memcpy_d2h(_ptr(a), C.c_void_p(self.ptr), a.nbytes, None), "d2h")
+        return a
+
+    def free(self):
+        if self.ptr:
+            lib().af_free(C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def sync():
+    check(lib().af_sync(), "af_sync")

@@ -1,0 +1,169 @@
+// af_internal.h -- shared definitions of the gfx950 dedispersion library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "astroflow_amd.h"
+
+namespace af {
+
+// A segment of binned rows [t0,t1) between consecutive IQRM window
+// boundaries; every row of a segment is covered by the same windows.
+struct Seg {
+  uint32_t t0, t1;
+  uint32_t w0, w1;  // covering windows [w0,w1)
+};
+
+// A block of rows [r0,r1) inside one segment: the unit of work of the
+// statistics and cleaning passes.
+struct RowItem {
+  uint32_t seg, r0, r1, pad;
+};
+
+// Per-(row block, channel) partial sums of the IQRM statistics pass.
+// Unsigned data: s = sum, (q_lo, q_hi) = 128-bit sum of squares.
+// float32 data: s and q_lo hold the bit patterns of double sums.
+struct Part {
+  uint64_t s, q_lo, q_hi;
+};
+
+constexpr int kDdLanesOut = 16;               // outputs per lane, u8 dedispersion
+constexpr int kDdTile = 64 * kDdLanesOut;     // outputs per wave tile
+constexpr int kDdRowsPerBlock = 4;            // DM rows (waves) per workgroup
+constexpr int kStatRows = 1024;               // rows per statistics item
+constexpr int kCleanRows = 64;                // rows per cleaning item
+
+size_t dtype_size(int dtype);
+int acc_size(int dtype);  // bytes of one output element
+
+// Error state (thread-local message).
+void set_error(const std::string &msg);
+af_status fail(af_status code, const std::string &msg);
+
+#define AF_HIP(call)                                                          \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      return ::af::fail(AF_ERR_RUNTIME, std::string(#call) + ": " +           \
+                                            hipGetErrorString(e_));           \
+  } while (0)
+
+// ----------------------------------------------------------------------
+// Kernel launchers (af_kernels.hip).  All are asynchronous on `st`.
+// ----------------------------------------------------------------------
+hipError_t launch_bin(int dtype, const void *in, void *out, size_t nsamp,
+                      int nchan, int ds, size_t nbin, hipStream_t st);
+
+hipError_t launch_seg_stats(int dtype, const void *x, int nchan, int c0, int c1,
+                            const RowItem *items, int nitems, Part *part,
+                            hipStream_t st);
+hipError_t launch_seg_reduce(int dtype, const Part *part, const int *seg_items,
+                             int nseg, int Csub, Part *segsum, hipStream_t st);
+hipError_t launch_iqrm_lags(int dtype, const Part *segsum, const Seg *segs,
+                            const uint32_t *wseg, const uint32_t *wt0,
+                            const uint32_t *wt1, int nwin, int Csub, int mode,
+                            const int *lags, int nlags, float nsigma,
+                            uint8_t *flags, size_t lds_bytes, hipStream_t st);
+hipError_t launch_iqrm_merge(const uint8_t *flags, const int *lags, int nlags,
+                             int nwin, int Csub, int c0, int nchan,
+                             uint8_t *masks, hipStream_t st);
+hipError_t launch_segmask(const uint8_t *masks, const Seg *segs, int nseg,
+                          int nchan, uint8_t *segmask, hipStream_t st);
+
+struct CleanArgs {
+  const void *x;            // [nbin][nchan] time-major
+  void *xT;                 // [dd_c1-dd_c0][Lp] channel-major
+  size_t Lp;
+  int nchan;
+  const RowItem *items;
+  int nitems;
+  const uint8_t *segmask;   // [nseg][nchan] or null
+  const uint8_t *stat_mask; // [nchan] or null
+  int zdm, zc0, zc1;
+  int dd_c0, dd_c1;
+};
+hipError_t launch_clean(int dtype, const CleanArgs &a, hipStream_t st);
+
+struct DdArgs {
+  const void *xT;           // [nch][Lp]
+  size_t Lp;
+  const int32_t *delays;    // [nrows][nch]
+  int nch;                  // channels in xT / delay row length
+  int nrows;                // DM rows
+  int ngrp;                 // channel groups (subbands); 1 = direct
+  int grp_w;                // channels per group
+  void *out;                // direct: [nrows][ostride]; grouped: [nrows][ngrp][ostride]
+  size_t ostride;
+  size_t nvalid;            // outputs written per row (direct: nout)
+  int ntiles;               // time tiles of kDdTile
+  int out16;                // 1: u16 output (u8 input, group width <= 256)
+};
+hipError_t launch_dedisperse(int dtype, const DdArgs &a, hipStream_t st);
+
+struct Stage2Args {
+  const void *inter;        // [ndm_nom][nsb][L1]
+  size_t L1;
+  int nsb;
+  const int32_t *res;       // [ndm][nsb]
+  const int32_t *maxres_j;  // [ndm_nom]
+  int ndm, ndm0, ndm_nom;
+  void *out;                // [ndm][nout]
+  size_t nout;
+  int tile;                 // outputs per workgroup tile
+  int lds_len;              // LDS row length (tile + max over groups of maxres)
+  int inter16;              // inter is u16 (u8 input)
+};
+hipError_t launch_stage2(int dtype, const Stage2Args &a, hipStream_t st);
+
+}  // namespace af
+
+// The opaque plan (C-ABI type).
+struct af_plan {
+  af_plan_desc d{};
+  std::vector<double> freq;
+  std::vector<int> bad;
+  int device = 0;
+
+  size_t nbin = 0, nout = 0, Lp = 0, L1 = 0;
+  int nch = 0, ndm = 0, nsb = 0, ndm_nom = 0, maxdelay = 0, maxres = 0;
+  int ntiles1 = 0;          // tiles of the first dedispersion stage
+  int s2_tile = 0, s2_lds = 0;
+  double adds = 0;
+  std::vector<int32_t> h_delays, h_res, h_maxres_j;
+
+  // IQRM / segmentation
+  int nwin = 0, nseg = 0, nlags = 0, Csub = 0;
+  size_t iqrm_lds = 0;
+  std::vector<uint32_t> wt0, wt1, wseg;
+  std::vector<af::Seg> segs;
+  std::vector<af::RowItem> stat_items, clean_items;
+  std::vector<int> seg_items, lags;
+
+  // device buffers
+  int32_t *d_delays = nullptr, *d_res = nullptr, *d_maxres_j = nullptr;
+  int *d_lags = nullptr, *d_seg_items = nullptr;
+  uint32_t *d_wseg = nullptr, *d_wt0 = nullptr, *d_wt1 = nullptr;
+  af::Seg *d_segs = nullptr;
+  af::RowItem *d_stat_items = nullptr, *d_clean_items = nullptr;
+  uint8_t *d_static = nullptr;
+  void *d_xb = nullptr, *d_xT = nullptr, *d_inter = nullptr;
+  af::Part *d_part = nullptr, *d_segsum = nullptr;
+  uint8_t *d_flags = nullptr, *d_masks = nullptr, *d_segmask = nullptr;
+  size_t scratch_bytes = 0;
+
+  hipStream_t stream = nullptr;
+  bool profiling = false;
+  // one set of kNumEv events per profiled execute, recorded between stages
+  static constexpr int kNumEv = 7;
+  std::vector<hipEvent_t> ev;   // pool, kNumEv per set
+  int ev_used = 0;              // sets recorded since the last collection
+  double acc_ms[kNumEv - 1] = {};
+  double acc_total = 0;
+  int acc_n = 0;
+  std::vector<void *> allocs;
+};

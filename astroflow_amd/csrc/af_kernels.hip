@@ -1,0 +1,723 @@
+// af_kernels.hip -- gfx950 kernels of the dedispersion + RFI path.
+//
+// Data layouts in HBM:
+//   input        x[nsamp][nchan]       time-major, as read from a filterbank
+//   binned       xb[nbin][nchan]       time-major (ds > 1 only)
+//   cleaned      xT[nch][Lp]           channel-major, channels [dd_c0, dd_c1),
+//                                      zero-padded from nbin to Lp so that the
+//                                      dedispersion loads need no bounds checks
+//   intermediate inter[ndm_nom][nsb][L1]  subband partial sums (u16 for u8 data)
+//   output       out[ndm][nout]        DM-major
+//
+// Compiled with -ffp-contract=off: every floating-point expression rounds
+// exactly as written, as the CPU oracle does.
+#include "af_internal.h"
+
+namespace af {
+
+// ----------------------------------------------------------------- helpers --
+
+template <typename T> struct Tr;
+template <> struct Tr<uint8_t> { static constexpr double vmax = 255.0; };
+template <> struct Tr<uint16_t> { static constexpr double vmax = 65535.0; };
+template <> struct Tr<uint32_t> { static constexpr double vmax = 4294967295.0; };
+template <> struct Tr<float> { static constexpr double vmax = 0.0; };
+
+// Block index remap so that consecutive logical blocks land on the same XCD
+// (the dispatcher hands block b to XCD b % 8): logical ranges stay inside one
+// XCD's L2.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
+  const uint32_t X = 8;
+  uint32_t q = n / X, r = n % X;
+  uint32_t xcd = b % X, k = b / X;
+  uint32_t start = xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
+  return start + k;
+}
+
+// Bytes 1 and 3 of w as two 16-bit lanes: (w >> 8) & 0x00FF00FF in one
+// v_perm_b32 (selector byte 0x0c yields zero).
+__device__ __forceinline__ uint32_t odd_bytes(uint32_t w) {
+  return __builtin_amdgcn_perm(0u, w, 0x0c030c01u);
+}
+
+__device__ __forceinline__ uint4 ldu4(const uint8_t *p) {
+  uint4 v;
+  __builtin_memcpy(&v, p, 16);
+  return v;
+}
+
+// Correctly rounded conversion of a 128-bit unsigned integer to double
+// (round to nearest, ties to even), as the host's (double)(unsigned __int128).
+__host__ __device__ inline double u128_to_double(uint64_t hi, uint64_t lo) {
+  if (hi == 0) return (double)lo;
+  int lz = __builtin_clzll(hi);
+  int sh = 64 - lz;  // bits of hi
+  // value = hi:lo ; keep the top 64 bits in m, remember the dropped bits
+  uint64_t m = (hi << lz) | (lo >> sh);
+  uint64_t dropped = lo << lz;  // bits shifted out below m (as a fraction)
+  // m has its top bit set; round m to 53 bits
+  uint64_t keep = m >> 11;
+  uint64_t rem = m & 0x7FF;
+  bool sticky = dropped != 0;
+  bool round_up = rem > 0x400 || (rem == 0x400 && (sticky || (keep & 1)));
+  if (round_up) keep += 1;
+  // keep is in [2^52, 2^53]; value = keep * 2^(11 + sh)
+  return ldexp((double)keep, 11 + sh);
+}
+
+// ------------------------------------------------------------------ binning --
+// src/core/dedispered.cu:337-376 (sqrt compression of the ds-sample sum).
+template <typename T>
+__global__ void k_bin(const T *__restrict__ in, T *__restrict__ out, size_t nsamp,
+                      int nchan, int ds, size_t nbin) {
+  const size_t total = nbin * (size_t)nchan;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    const size_t c = idx % (size_t)nchan, s = idx / (size_t)nchan;
+    const size_t t0 = s * (size_t)ds;
+    const size_t t1 = t0 + (size_t)ds < nsamp ? t0 + (size_t)ds : nsamp;
+    if constexpr (std::is_same<T, float>::value) {
+      double acc = 0.0;
+      for (size_t t = t0; t < t1; ++t) acc += (double)in[t * nchan + c];
+      out[idx] = (float)acc;
+    } else {
+      uint64_t sum = 0;
+      for (size_t t = t0; t < t1; ++t) sum += (uint64_t)in[t * nchan + c];
+      const double vmax = Tr<T>::vmax;
+      const uint64_t cnt = (uint64_t)(t1 - t0);
+      double sq = sqrt((double)sum);
+      double smax = sqrt((double)((uint64_t)vmax * cnt));
+      double v = (sq / smax) * vmax;
+      double r = v < vmax ? v : vmax;
+      out[idx] = (T)r;
+    }
+  }
+}
+
+// ------------------------------------------------------- IQRM statistics --
+// Per (row block, channel) sums over the block's rows: the first step of
+// src/core/iqrm.cu:21-69, with exact integer sums (coalesced: a wave reads 64
+// consecutive channels of a row).
+template <typename T>
+__global__ __launch_bounds__(256) void k_seg_stats(const T *__restrict__ x, int nchan,
+                                                   int c0, int c1,
+                                                   const RowItem *__restrict__ items,
+                                                   Part *__restrict__ part) {
+  const int Csub = c1 - c0;
+  const int nchunk = (Csub + 255) / 256;
+  const int item = blockIdx.x / nchunk;
+  const int c = c0 + (blockIdx.x % nchunk) * 256 + threadIdx.x;
+  if (c >= c1) return;
+  const RowItem it = items[item];
+  Part p{0, 0, 0};
+  if constexpr (std::is_same<T, float>::value) {
+    double s = 0.0, q = 0.0;
+    for (uint32_t r = it.r0; r < it.r1; ++r) {
+      double v = (double)x[(size_t)r * nchan + c];
+      s += v;
+      q += v * v;
+    }
+    p.s = __double_as_longlong(s);
+    p.q_lo = __double_as_longlong(q);
+  } else {
+    uint64_t s = 0, lo = 0, hi = 0;
+    for (uint32_t r = it.r0; r < it.r1; ++r) {
+      uint64_t v = (uint64_t)x[(size_t)r * nchan + c];
+      uint64_t q = v * v;
+      s += v;
+      lo += q;
+      hi += (lo < q) ? 1u : 0u;
+    }
+    p.s = s;
+    p.q_lo = lo;
+    p.q_hi = hi;
+  }
+  part[(size_t)item * Csub + (c - c0)] = p;
+}
+
+// Sum the row-block partials of each segment in item order.
+template <typename T>
+__global__ void k_seg_reduce(const Part *__restrict__ part, const int *__restrict__ seg_items,
+                             int nseg, int Csub, Part *__restrict__ segsum) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)nseg * Csub) return;
+  const int s = (int)(idx / Csub), c = (int)(idx % Csub);
+  Part acc{0, 0, 0};
+  if constexpr (std::is_same<T, float>::value) {
+    double a = 0.0, b = 0.0;
+    for (int i = seg_items[s]; i < seg_items[s + 1]; ++i) {
+      a += __longlong_as_double(part[(size_t)i * Csub + c].s);
+      b += __longlong_as_double(part[(size_t)i * Csub + c].q_lo);
+    }
+    acc.s = __double_as_longlong(a);
+    acc.q_lo = __double_as_longlong(b);
+  } else {
+    for (int i = seg_items[s]; i < seg_items[s + 1]; ++i) {
+      const Part p = part[(size_t)i * Csub + c];
+      acc.s += p.s;
+      uint64_t lo = acc.q_lo + p.q_lo;
+      acc.q_hi += p.q_hi + (lo < p.q_lo ? 1u : 0u);
+      acc.q_lo = lo;
+    }
+  }
+  segsum[(size_t)s * Csub + c] = acc;
+}
+
+// --------------------------------------------------- IQRM robust voting --
+// One workgroup per (window, lag): statistic from the window's segment sums
+// (src/core/iqrm.cu:72-91), lagged difference with edge clipping (:94-117),
+// quartiles by a bitonic sort in LDS and linear interpolation in double
+// (:288-306), and the flag test (:120-135).
+template <typename T>
+__global__ __launch_bounds__(1024) void k_iqrm_lag(
+    const Part *__restrict__ segsum, const uint32_t *__restrict__ wseg,
+    const uint32_t *__restrict__ wt0, const uint32_t *__restrict__ wt1, int Csub,
+    int mode, const int *__restrict__ lags, int nlags, float nsigma,
+    uint8_t *__restrict__ flags) {
+  extern __shared__ float lds[];
+  const int w = blockIdx.x / nlags, li = blockIdx.x % nlags;
+  const int lag = lags[li];
+  int P2 = 1;
+  while (P2 < Csub) P2 <<= 1;
+  float *xs = lds;
+  float *ds_ = lds + Csub;
+  float *srt = lds + 2 * Csub;
+  __shared__ float sh_med, sh_thr;
+  __shared__ int sh_valid;
+  uint8_t *frow = flags + ((size_t)w * nlags + li) * Csub;
+  const bool lag_ok = lag < Csub && -lag < Csub;
+  if (!lag_ok) {
+    for (int k = threadIdx.x; k < Csub; k += blockDim.x) frow[k] = 0;
+    return;
+  }
+  const uint32_t s0 = wseg[2 * w], s1 = wseg[2 * w + 1];
+  const double W = (double)(wt1[w] - wt0[w]);
+  for (int c = threadIdx.x; c < Csub; c += blockDim.x) {
+    float xv;
+    if constexpr (std::is_same<T, float>::value) {
+      double S = 0.0, Q = 0.0;
+      for (uint32_t s = s0; s < s1; ++s) {
+        S += __longlong_as_double(segsum[(size_t)s * Csub + c].s);
+        Q += __longlong_as_double(segsum[(size_t)s * Csub + c].q_lo);
+      }
+      double mean = S / W;
+      if (mode == 0) {
+        xv = (float)mean;
+      } else {
+        double var = Q / W - mean * mean;
+        if (var < 0.0) var = 0.0;
+        xv = (float)sqrt(var);
+      }
+    } else {
+      uint64_t S = 0, lo = 0, hi = 0;
+      for (uint32_t s = s0; s < s1; ++s) {
+        const Part p = segsum[(size_t)s * Csub + c];
+        S += p.s;
+        uint64_t nl = lo + p.q_lo;
+        hi += p.q_hi + (nl < p.q_lo ? 1u : 0u);
+        lo = nl;
+      }
+      double mean = (double)S / W;
+      if (mode == 0) {
+        xv = (float)mean;
+      } else {
+        double var = u128_to_double(hi, lo) / W - mean * mean;
+        if (var < 0.0) var = 0.0;
+        xv = (float)sqrt(var);
+      }
+    }
+    xs[c] = xv;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < P2; k += blockDim.x) {
+    float dv = INFINITY;
+    if (k < Csub) {
+      if (lag < 0) {
+        const int cut = Csub + lag;
+        dv = (k < cut) ? xs[k] - xs[k - lag] : xs[cut] - xs[Csub - 1];
+      } else {
+        dv = (k < lag) ? xs[lag] - xs[0] : xs[k] - xs[k - lag];
+      }
+      ds_[k] = dv;
+    }
+    srt[k] = dv;
+  }
+  __syncthreads();
+  // bitonic sort, ascending
+  const int half = P2 >> 1;
+  for (int kk = 2; kk <= P2; kk <<= 1) {
+    for (int j = kk >> 1; j > 0; j >>= 1) {
+      for (int p = threadIdx.x; p < half; p += blockDim.x) {
+        const int i = 2 * j * (p / j) + (p % j);
+        const int q = i + j;
+        const bool asc = (i & kk) == 0;
+        float a = srt[i], b = srt[q];
+        if ((a > b) == asc) {
+          srt[i] = b;
+          srt[q] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (threadIdx.x == 0) {
+    float qv[3];
+    const float qs[3] = {0.25f, 0.50f, 0.75f};
+    for (int i = 0; i < 3; ++i) {
+      if (Csub == 1) {
+        qv[i] = srt[0];
+      } else {
+        double pos = (double)qs[i] * (double)(Csub - 1);
+        int lo = (int)floor(pos), hi = (int)ceil(pos);
+        double wgt = pos - (double)lo;
+        float diff = srt[hi] - srt[lo];
+        double prod = (double)diff * wgt;
+        qv[i] = (float)((double)srt[lo] + prod);
+      }
+    }
+    float s = fabsf((qv[2] - qv[0]) / 1.349f);
+    sh_valid = (s > 0.f) && isfinite(s);
+    sh_med = qv[1];
+    sh_thr = nsigma * s;
+  }
+  __syncthreads();
+  const int valid = sh_valid;
+  const float med = sh_med, thr = sh_thr;
+  for (int k = threadIdx.x; k < Csub; k += blockDim.x) {
+    float val = ds_[k] - med;
+    frow[k] = (valid && val > thr) ? 1 : 0;
+  }
+}
+
+__device__ __forceinline__ int iqrm_partner(int k, int lag, int C) {
+  if (lag < 0) return (k < C + lag) ? k - lag : C - 1;
+  return (k < lag) ? 0 : k - lag;
+}
+
+// Votes and final window mask (src/core/iqrm.cu:138-185).
+__global__ __launch_bounds__(1024) void k_iqrm_merge(const uint8_t *__restrict__ flags,
+                                                     const int *__restrict__ lags,
+                                                     int nlags, int Csub, int c0,
+                                                     int nchan, uint8_t *__restrict__ masks) {
+  extern __shared__ int cast[];
+  const int w = blockIdx.x;
+  const uint8_t *f = flags + (size_t)w * nlags * Csub;
+  for (int k = threadIdx.x; k < Csub; k += blockDim.x) cast[k] = 0;
+  __syncthreads();
+  for (int k = threadIdx.x; k < Csub; k += blockDim.x)
+    for (int l = 0; l < nlags; ++l)
+      if (f[(size_t)l * Csub + k]) atomicAdd(&cast[iqrm_partner(k, lags[l], Csub)], 1);
+  __syncthreads();
+  uint8_t *mrow = masks + (size_t)w * nchan;
+  for (int c = threadIdx.x; c < nchan; c += blockDim.x) {
+    const int k = c - c0;
+    uint8_t m = 0;
+    if (k >= 0 && k < Csub) {
+      int recv = 0;
+      for (int l = 0; l < nlags; ++l) recv += f[(size_t)l * Csub + k];
+      if (recv > 0)
+        for (int l = 0; l < nlags; ++l)
+          if (f[(size_t)l * Csub + k] && cast[iqrm_partner(k, lags[l], Csub)] < recv) {
+            m = 1;
+            break;
+          }
+    }
+    mrow[c] = m;
+  }
+}
+
+// OR of the masks of the windows covering each segment.
+__global__ void k_segmask(const uint8_t *__restrict__ masks, const Seg *__restrict__ segs,
+                          int nseg, int nchan, uint8_t *__restrict__ segmask) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)nseg * nchan) return;
+  const int s = (int)(idx / nchan), c = (int)(idx % nchan);
+  uint8_t m = 0;
+  for (uint32_t w = segs[s].w0; w < segs[s].w1; ++w) m |= masks[(size_t)w * nchan + c];
+  segmask[idx] = m;
+}
+
+// --------------------------------------------- clean + transpose pass --
+// Per row block: IQRM window mask, zero-DM (src/rfi/rfimarker.cu:172-254:
+// mean of the non-zero values over [zc0,zc1), x = clamp(x - mean, 0, vmax)),
+// static bad channels (src/rfi/rfimarker.cu:9-30), then a transpose through
+// LDS into the channel-major buffer the dedispersion kernels read.
+template <typename T>
+__device__ __forceinline__ T zdm_sub(T v, float m) {
+  if constexpr (std::is_same<T, float>::value) {
+    return v - m;
+  } else {
+    float r = (float)v - m;
+    if (r < 0.0f) r = 0.0f;
+    const float vmax = (float)Tr<T>::vmax;
+    if (r > vmax) r = vmax;
+    if (r >= 4294967296.0f) return (T)0xFFFFFFFFu;
+    return (T)(uint32_t)r;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_clean(CleanArgs a) {
+  __shared__ float m_s[kCleanRows];
+  __shared__ T tile[64][kCleanRows + 4];
+  const RowItem it = a.items[blockIdx.x];
+  const int nrows = (int)(it.r1 - it.r0);
+  const T *__restrict__ x = (const T *)a.x;
+  T *__restrict__ xT = (T *)a.xT;
+  const uint8_t *smask = a.segmask ? a.segmask + (size_t)it.seg * a.nchan : nullptr;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (a.zdm) {
+    for (int rr = wave; rr < nrows; rr += 4) {
+      const T *row = x + (size_t)(it.r0 + rr) * a.nchan;
+      if constexpr (std::is_same<T, float>::value) {
+        double S = 0.0;
+        unsigned n = 0;
+        for (int c = a.zc0 + lane; c < a.zc1; c += 64) {
+          float v = row[c];
+          if (smask && smask[c]) v = 0.0f;
+          if (v != 0.0f) { S += (double)v; ++n; }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+          S += __shfl_xor(S, o);
+          n += __shfl_xor(n, o);
+        }
+        if (lane == 0) m_s[rr] = n ? (float)(S / (double)n) : 0.0f;
+      } else {
+        uint64_t S = 0;
+        unsigned n = 0;
+        for (int c = a.zc0 + lane; c < a.zc1; c += 64) {
+          T v = row[c];
+          if (smask && smask[c]) v = 0;
+          if (v != 0) { S += (uint64_t)v; ++n; }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+          S += __shfl_xor(S, o);
+          n += __shfl_xor(n, o);
+        }
+        if (lane == 0) m_s[rr] = n ? (float)S / (float)n : 0.0f;
+      }
+    }
+    __syncthreads();
+  }
+  for (int cb = a.dd_c0; cb < a.dd_c1; cb += 64) {
+    const int c = cb + lane;
+    for (int rr = wave; rr < kCleanRows; rr += 4) {
+      T v = 0;
+      if (c < a.dd_c1 && rr < nrows) {
+        v = x[(size_t)(it.r0 + rr) * a.nchan + c];
+        if (smask && smask[c]) v = 0;
+        if (a.zdm && c >= a.zc0 && c < a.zc1) v = zdm_sub<T>(v, m_s[rr]);
+        if (a.stat_mask && a.stat_mask[c]) v = 0;
+      }
+      tile[lane][rr] = v;
+    }
+    __syncthreads();
+    for (int k = wave; k < 64; k += 4) {
+      const int cc = cb + k;
+      if (cc < a.dd_c1 && lane < nrows)
+        xT[(size_t)(cc - a.dd_c0) * a.Lp + it.r0 + lane] = tile[k][lane];
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------- dedispersion --
+// u8 data: each wave owns one DM row and 1024 consecutive outputs (16 per
+// lane); per channel every lane loads 16 consecutive bytes of the channel-
+// major buffer at its delay (one unaligned 16-byte load; the delay is
+// wave-uniform and comes from a scalar load).  The bytes are summed as
+// packed 16-bit lanes: S += w accumulates the raw words, H += (w>>8)&0x00FF00FF
+// the odd bytes; after at most 256 channels the even-byte sums are
+// S - (H<<8).  That is two adds per four samples (three with the byte
+// extraction), exact because no 16-bit lane can exceed 256*255.
+__global__ __launch_bounds__(256) void k_dd_u8(DdArgs a) {
+  const int nrowblk = (a.nrows + kDdRowsPerBlock - 1) / kDdRowsPerBlock;
+  const uint32_t lid = xcd_remap(blockIdx.x, gridDim.x);
+  const int rowblk = (int)(lid % (uint32_t)nrowblk);
+  const uint32_t rest = lid / (uint32_t)nrowblk;
+  const int tile = (int)(rest % (uint32_t)a.ntiles);
+  const int grp = (int)(rest / (uint32_t)a.ntiles);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int row = rowblk * kDdRowsPerBlock + wave;
+  if (row >= a.nrows) return;
+  const int ca = grp * a.grp_w;
+  const int cb = min(ca + a.grp_w, a.nch);
+  const size_t t = (size_t)tile * kDdTile + (size_t)lane * kDdLanesOut;
+  // uniform (scalar) part of the address: tile start; per-lane part: 32-bit offset
+  const uint8_t *__restrict__ tbase = (const uint8_t *)a.xT + (size_t)tile * kDdTile;
+  const uint32_t loff = (uint32_t)lane * kDdLanesOut;
+  const int32_t *__restrict__ drow = a.delays + (size_t)row * a.nch;
+
+  uint32_t tot[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) tot[i] = 0;
+
+  for (int c0 = ca; c0 < cb; c0 += 256) {
+    const int c1 = min(c0 + 256, cb);
+    uint32_t S0 = 0, S1 = 0, S2 = 0, S3 = 0, H0 = 0, H1 = 0, H2 = 0, H3 = 0;
+#pragma unroll 4
+    for (int c = c0; c < c1; ++c) {
+      const uint8_t *rp = tbase + (size_t)c * a.Lp + drow[c];
+      const uint4 w = ldu4(rp + loff);
+      S0 += w.x; S1 += w.y; S2 += w.z; S3 += w.w;
+      H0 += odd_bytes(w.x);
+      H1 += odd_bytes(w.y);
+      H2 += odd_bytes(w.z);
+      H3 += odd_bytes(w.w);
+    }
+    const uint32_t L0 = S0 - (H0 << 8), L1 = S1 - (H1 << 8);
+    const uint32_t L2 = S2 - (H2 << 8), L3 = S3 - (H3 << 8);
+    tot[0] += L0 & 0xFFFFu;  tot[1] += H0 & 0xFFFFu;  tot[2] += L0 >> 16;  tot[3] += H0 >> 16;
+    tot[4] += L1 & 0xFFFFu;  tot[5] += H1 & 0xFFFFu;  tot[6] += L1 >> 16;  tot[7] += H1 >> 16;
+    tot[8] += L2 & 0xFFFFu;  tot[9] += H2 & 0xFFFFu;  tot[10] += L2 >> 16; tot[11] += H2 >> 16;
+    tot[12] += L3 & 0xFFFFu; tot[13] += H3 & 0xFFFFu; tot[14] += L3 >> 16; tot[15] += H3 >> 16;
+  }
+
+  const size_t orow = ((size_t)row * a.ngrp + grp) * a.ostride;
+  if (a.out16) {
+    uint16_t *o = (uint16_t *)a.out + orow + t;
+    uint4 v0 = make_uint4(tot[0] | (tot[1] << 16), tot[2] | (tot[3] << 16),
+                          tot[4] | (tot[5] << 16), tot[6] | (tot[7] << 16));
+    uint4 v1 = make_uint4(tot[8] | (tot[9] << 16), tot[10] | (tot[11] << 16),
+                          tot[12] | (tot[13] << 16), tot[14] | (tot[15] << 16));
+    if (t + 16 <= a.nvalid) {
+      __builtin_memcpy(o, &v0, 16);
+      __builtin_memcpy(o + 8, &v1, 16);
+    } else {
+      for (int i = 0; i < 16; ++i)
+        if (t + i < a.nvalid) o[i] = (uint16_t)tot[i];
+    }
+  } else {
+    uint32_t *o = (uint32_t *)a.out + orow + t;
+    if (t + 16 <= a.nvalid) {
+      __builtin_memcpy(o, &tot[0], 64);
+    } else {
+      for (int i = 0; i < 16; ++i)
+        if (t + i < a.nvalid) o[i] = tot[i];
+    }
+  }
+}
+
+// Element-wise dedispersion for u16/u32/f32 data (accumulator Acc, output
+// OutT), same tiling as k_dd_u8.
+template <typename T, typename Acc, typename OutT>
+__global__ __launch_bounds__(256) void k_dd_generic(DdArgs a) {
+  const int nrowblk = (a.nrows + kDdRowsPerBlock - 1) / kDdRowsPerBlock;
+  const uint32_t lid = xcd_remap(blockIdx.x, gridDim.x);
+  const int rowblk = (int)(lid % (uint32_t)nrowblk);
+  const uint32_t rest = lid / (uint32_t)nrowblk;
+  const int tile = (int)(rest % (uint32_t)a.ntiles);
+  const int grp = (int)(rest / (uint32_t)a.ntiles);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int row = rowblk * kDdRowsPerBlock + wave;
+  if (row >= a.nrows) return;
+  const int ca = grp * a.grp_w;
+  const int cb = min(ca + a.grp_w, a.nch);
+  const size_t t = (size_t)tile * kDdTile + (size_t)lane * kDdLanesOut;
+  const T *__restrict__ base = (const T *)a.xT + t;
+  const int32_t *__restrict__ drow = a.delays + (size_t)row * a.nch;
+  Acc acc[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) acc[i] = 0;
+  for (int c = ca; c < cb; ++c) {
+    const T *p = base + (size_t)c * a.Lp + drow[c];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] += (Acc)p[i];
+  }
+  OutT *o = (OutT *)a.out + ((size_t)row * a.ngrp + grp) * a.ostride + t;
+  for (int i = 0; i < 16; ++i)
+    if (t + i < a.nvalid) o[i] = (OutT)acc[i];
+}
+
+// Subband stage 2 for u16 partial sums (src/core/dedispered.cu:449-476):
+// out[m][t] = sum_sb inter[j][sb][t + res[m][sb]], j = m / ndm0.  A workgroup
+// stages the group's partial sums for its tile (tile + max residual of the
+// group) in LDS, then each wave sums the subbands for one fine DM at a time.
+__global__ __launch_bounds__(256) void k_stage2_u16(Stage2Args a) {
+  extern __shared__ uint16_t s2[];
+  const int ntile = (int)((a.nout + a.tile - 1) / a.tile);
+  const uint32_t lid = xcd_remap(blockIdx.x, gridDim.x);
+  const int ti = (int)(lid % (uint32_t)ntile);
+  const int j = (int)(lid / (uint32_t)ntile);
+  const size_t t0 = (size_t)ti * a.tile;
+  const int ext = a.tile + a.maxres_j[j];
+  const uint16_t *__restrict__ src = (const uint16_t *)a.inter + (size_t)j * a.nsb * a.L1 + t0;
+  for (int sb = 0; sb < a.nsb; ++sb)
+    for (int i = threadIdx.x; i < ext; i += blockDim.x)
+      s2[sb * a.lds_len + i] = src[(size_t)sb * a.L1 + i];
+  __syncthreads();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int per = a.tile / 64;  // outputs per lane (tile is a multiple of 64)
+  for (int mi = wave; mi < a.ndm0; mi += 4) {
+    const int m = j * a.ndm0 + mi;
+    if (m >= a.ndm) break;
+    const int32_t *__restrict__ rrow = a.res + (size_t)m * a.nsb;
+    uint32_t acc[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = 0;
+    for (int sb = 0; sb < a.nsb; ++sb) {
+      const uint16_t *p = s2 + sb * a.lds_len + lane * per + rrow[sb];
+#pragma unroll
+      for (int i = 0; i < 16; ++i)
+        if (i < per) acc[i] += p[i];
+    }
+    uint32_t *o = (uint32_t *)a.out + (size_t)m * a.nout + t0 + (size_t)lane * per;
+    for (int i = 0; i < per; ++i)
+      if (t0 + (size_t)lane * per + i < a.nout) o[i] = acc[i];
+  }
+}
+
+// Stage 2 for u32/u64/double partial sums, straight from HBM/L2.
+template <typename Acc, typename OutT>
+__global__ __launch_bounds__(256) void k_stage2_generic(Stage2Args a) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)a.ndm * a.nout) return;
+  const int m = (int)(idx / a.nout);
+  const size_t t = idx % a.nout;
+  const int j = m / a.ndm0;
+  const Acc *src = (const Acc *)a.inter + (size_t)j * a.nsb * a.L1 + t;
+  Acc acc = 0;
+  for (int sb = 0; sb < a.nsb; ++sb) acc += src[(size_t)sb * a.L1 + a.res[(size_t)m * a.nsb + sb]];
+  ((OutT *)a.out)[idx] = (OutT)acc;
+}
+
+// ---------------------------------------------------------- launchers --
+
+static unsigned grid_for(size_t n, unsigned bs, unsigned cap = 1u << 20) {
+  size_t g = (n + bs - 1) / bs;
+  if (g == 0) g = 1;
+  return (unsigned)(g < cap ? g : cap);
+}
+
+#define AF_DISPATCH(dtype, T, ...)                                            \
+  switch (dtype) {                                                            \
+  case AF_U8: { using T = uint8_t; __VA_ARGS__; break; }                      \
+  case AF_U16: { using T = uint16_t; __VA_ARGS__; break; }                    \
+  case AF_U32: { using T = uint32_t; __VA_ARGS__; break; }                    \
+  case AF_F32: { using T = float; __VA_ARGS__; break; }                       \
+  default: return hipErrorInvalidValue;                                       \
+  }
+
+hipError_t launch_bin(int dtype, const void *in, void *out, size_t nsamp, int nchan,
+                      int ds, size_t nbin, hipStream_t st) {
+  const size_t n = nbin * (size_t)nchan;
+  AF_DISPATCH(dtype, T,
+              hipLaunchKernelGGL(k_bin<T>, dim3(grid_for(n, 256)), dim3(256), 0, st,
+                                 (const T *)in, (T *)out, nsamp, nchan, ds, nbin));
+  return hipGetLastError();
+}
+
+hipError_t launch_seg_stats(int dtype, const void *x, int nchan, int c0, int c1,
+                            const RowItem *items, int nitems, Part *part, hipStream_t st) {
+  const int nchunk = (c1 - c0 + 255) / 256;
+  AF_DISPATCH(dtype, T,
+              hipLaunchKernelGGL(k_seg_stats<T>, dim3((unsigned)(nitems * nchunk)), dim3(256),
+                                 0, st, (const T *)x, nchan, c0, c1, items, part));
+  return hipGetLastError();
+}
+
+hipError_t launch_seg_reduce(int dtype, const Part *part, const int *seg_items, int nseg,
+                             int Csub, Part *segsum, hipStream_t st) {
+  const size_t n = (size_t)nseg * Csub;
+  AF_DISPATCH(dtype, T,
+              hipLaunchKernelGGL(k_seg_reduce<T>, dim3((unsigned)((n + 255) / 256)), dim3(256),
+                                 0, st, part, seg_items, nseg, Csub, segsum));
+  return hipGetLastError();
+}
+
+hipError_t launch_iqrm_lags(int dtype, const Part *segsum, const Seg *segs,
+                            const uint32_t *wseg, const uint32_t *wt0, const uint32_t *wt1,
+                            int nwin, int Csub, int mode, const int *lags, int nlags,
+                            float nsigma, uint8_t *flags, size_t lds_bytes, hipStream_t st) {
+  (void)segs;
+  AF_DISPATCH(dtype, T,
+              hipLaunchKernelGGL(k_iqrm_lag<T>, dim3((unsigned)(nwin * nlags)), dim3(1024),
+                                 lds_bytes, st, segsum, wseg, wt0, wt1, Csub, mode, lags,
+                                 nlags, nsigma, flags));
+  return hipGetLastError();
+}
+
+hipError_t launch_iqrm_merge(const uint8_t *flags, const int *lags, int nlags, int nwin,
+                             int Csub, int c0, int nchan, uint8_t *masks, hipStream_t st) {
+  hipLaunchKernelGGL(k_iqrm_merge, dim3((unsigned)nwin), dim3(1024),
+                     (size_t)Csub * sizeof(int), st, flags, lags, nlags, Csub, c0, nchan, masks);
+  return hipGetLastError();
+}
+
+hipError_t launch_segmask(const uint8_t *masks, const Seg *segs, int nseg, int nchan,
+                          uint8_t *segmask, hipStream_t st) {
+  const size_t n = (size_t)nseg * nchan;
+  hipLaunchKernelGGL(k_segmask, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, masks,
+                     segs, nseg, nchan, segmask);
+  return hipGetLastError();
+}
+
+hipError_t launch_clean(int dtype, const CleanArgs &a, hipStream_t st) {
+  AF_DISPATCH(dtype, T,
+              hipLaunchKernelGGL(k_clean<T>, dim3((unsigned)a.nitems), dim3(256), 0, st, a));
+  return hipGetLastError();
+}
+
+hipError_t launch_dedisperse(int dtype, const DdArgs &a, hipStream_t st) {
+  const int nrowblk = (a.nrows + kDdRowsPerBlock - 1) / kDdRowsPerBlock;
+  const size_t nblk = (size_t)nrowblk * a.ntiles * a.ngrp;
+  if (nblk == 0) return hipSuccess;
+  if (nblk > 0x7FFFFFFFull) return hipErrorInvalidValue;
+  const dim3 g((unsigned)nblk), b(256);
+  switch (dtype) {
+  case AF_U8:
+    if (a.out16 || a.ngrp == 1) {
+      hipLaunchKernelGGL(k_dd_u8, g, b, 0, st, a);
+    } else {
+      hipLaunchKernelGGL((k_dd_generic<uint8_t, uint32_t, uint32_t>), g, b, 0, st, a);
+    }
+    break;
+  case AF_U16:
+    hipLaunchKernelGGL((k_dd_generic<uint16_t, uint32_t, uint32_t>), g, b, 0, st, a);
+    break;
+  case AF_U32:
+    hipLaunchKernelGGL((k_dd_generic<uint32_t, uint64_t, uint64_t>), g, b, 0, st, a);
+    break;
+  case AF_F32:
+    if (a.ngrp == 1)
+      hipLaunchKernelGGL((k_dd_generic<float, double, float>), g, b, 0, st, a);
+    else
+      hipLaunchKernelGGL((k_dd_generic<float, double, double>), g, b, 0, st, a);
+    break;
+  default:
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_stage2(int dtype, const Stage2Args &a, hipStream_t st) {
+  if (dtype == AF_U8 && a.inter16) {
+    const int ntile = (int)((a.nout + a.tile - 1) / a.tile);
+    const size_t nblk = (size_t)ntile * a.ndm_nom;
+    const size_t lds = (size_t)a.nsb * a.lds_len * sizeof(uint16_t);
+    hipLaunchKernelGGL(k_stage2_u16, dim3((unsigned)nblk), dim3(256), lds, st, a);
+    return hipGetLastError();
+  }
+  const size_t n = (size_t)a.ndm * a.nout;
+  const dim3 g((unsigned)((n + 255) / 256)), b(256);
+  switch (dtype) {
+  case AF_U8:
+  case AF_U16:
+    hipLaunchKernelGGL((k_stage2_generic<uint32_t, uint32_t>), g, b, 0, st, a);
+    break;
+  case AF_U32:
+    hipLaunchKernelGGL((k_stage2_generic<uint64_t, uint64_t>), g, b, 0, st, a);
+    break;
+  case AF_F32:
+    hipLaunchKernelGGL((k_stage2_generic<double, float>), g, b, 0, st, a);
+    break;
+  default:
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace af

@@ -1,0 +1,728 @@
+// af_plan.hip -- host side of the C-ABI: plans (tables, segmentation, device
+// memory), the execute sequence and the device helpers.
+//
+// Built with -ffp-contract=off: the FP64 delay and residual tables round
+// exactly as the expressions are written (the CPU oracle uses the same
+// expressions).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "af_internal.h"
+
+namespace af {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+af_status fail(af_status code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+size_t dtype_size(int dtype) {
+  switch (dtype) {
+  case AF_U8: return 1;
+  case AF_U16: return 2;
+  case AF_U32: return 4;
+  case AF_F32: return 4;
+  default: return 0;
+  }
+}
+
+int acc_size(int dtype) { return dtype == AF_U32 ? 8 : 4; }
+
+static constexpr double kDM = 4148.808;  // src/core/dedispered.cu:333
+
+// Delay in native samples: round half away from zero of
+// K*dm*(f^-2 - fref^-2)/tsamp (src/core/dedispered.cu:329-334, in FP64).
+static long delay_native(double f, double fref, double dm, double tsamp) {
+  double inv = 1.0 / (f * f) - 1.0 / (fref * fref);
+  double delay = kDM * dm * inv;
+  return (long)std::round(delay / tsamp);
+}
+
+// Subband residual (src/core/dedispered.cu:384-413, in FP64).
+static long residual(double f, double fref, double ddm, double tsamp, int ds) {
+  double inv = 1.0 / (f * f) - 1.0 / (fref * fref);
+  double delay = kDM * ddm * inv;
+  double bins = delay / (tsamp * (double)ds);
+  return (long)std::floor(bins + 0.5);
+}
+
+// IQRM windows (src/core/iqrm.cu:368-395).
+static void make_windows(size_t n, unsigned W, unsigned H, bool tail,
+                         std::vector<uint32_t> &t0s, std::vector<uint32_t> &t1s) {
+  t0s.clear();
+  t1s.clear();
+  if (n == 0) return;
+  if (W == 0 || W >= n) {
+    t0s.push_back(0);
+    t1s.push_back((uint32_t)n);
+    return;
+  }
+  unsigned t0 = 0;
+  while ((size_t)t0 + W <= n) {
+    t0s.push_back(t0);
+    t1s.push_back(t0 + W);
+    t0 += (H ? H : W);
+  }
+  if (tail && t0 < n && n > (size_t)t0 + 4) {
+    t0s.push_back(t0);
+    t1s.push_back((uint32_t)n);
+  }
+}
+
+// IQRM lags (src/core/iqrm.cu:189-200).
+static std::vector<int> make_lags(int radius, double geofactor) {
+  std::vector<int> lags;
+  if (radius <= 0) return lags;
+  int lag = 1;
+  while (lag <= radius) {
+    lags.push_back(+lag);
+    lags.push_back(-lag);
+    int nxt = (int)(geofactor * lag);
+    lag = std::max(nxt, lag + 1);
+  }
+  return lags;
+}
+
+template <typename V>
+static af_status upload(af_plan *p, V **dst, const std::vector<V> &src) {
+  if (src.empty()) {
+    *dst = nullptr;
+    return AF_OK;
+  }
+  void *q = nullptr;
+  hipError_t e = hipMalloc(&q, src.size() * sizeof(V));
+  if (e != hipSuccess) return fail(AF_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  p->allocs.push_back(q);
+  p->scratch_bytes += src.size() * sizeof(V);
+  AF_HIP(hipMemcpy(q, src.data(), src.size() * sizeof(V), hipMemcpyHostToDevice));
+  *dst = (V *)q;
+  return AF_OK;
+}
+
+static af_status dalloc(af_plan *p, void **dst, size_t bytes) {
+  if (bytes == 0) {
+    *dst = nullptr;
+    return AF_OK;
+  }
+  void *q = nullptr;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess)
+    return fail(AF_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes: " +
+                                  hipGetErrorString(e));
+  p->allocs.push_back(q);
+  p->scratch_bytes += bytes;
+  *dst = q;
+  return AF_OK;
+}
+
+static void release(af_plan *p) {
+  for (void *q : p->allocs) (void)hipFree(q);
+  p->allocs.clear();
+  for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
+  p->ev.clear();
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  p->stream = nullptr;
+}
+
+#define AF_TRY(x)                                                             \
+  do {                                                                        \
+    af_status s_ = (x);                                                       \
+    if (s_ != AF_OK) return s_;                                               \
+  } while (0)
+
+static af_status validate(const af_plan_desc &d) {
+  if (d.dtype < AF_U8 || d.dtype > AF_F32) return fail(AF_ERR_INVALID, "unknown dtype");
+  if (d.nchan <= 0) return fail(AF_ERR_INVALID, "nchan must be > 0");
+  if (d.nsamp == 0) return fail(AF_ERR_INVALID, "nsamp must be > 0");
+  if (d.ds < 1) return fail(AF_ERR_INVALID, "time_downsample must be >= 1");
+  if (!(d.tsamp > 0.0)) return fail(AF_ERR_INVALID, "tsamp must be > 0");
+  if (!d.freq_mhz) return fail(AF_ERR_INVALID, "frequency table missing");
+  if (d.dd_c0 < 0 || d.dd_c1 > d.nchan || d.dd_c0 >= d.dd_c1)
+    return fail(AF_ERR_INVALID, "invalid dedispersion channel range");
+  if (d.ndm < 1) return fail(AF_ERR_INVALID, "ndm must be >= 1");
+  if (d.dm_low < 0.0) return fail(AF_ERR_INVALID, "dm_low must be >= 0");
+  if (d.ndm > 1 && !(d.dm_step > 0.0)) return fail(AF_ERR_INVALID, "dm_step must be > 0");
+  if (d.method != AF_METHOD_DIRECT && d.method != AF_METHOD_SUBBAND)
+    return fail(AF_ERR_INVALID, "unknown method");
+  if (d.method == AF_METHOD_SUBBAND && (d.sb_nchan < 1 || d.ndm0 < 1))
+    return fail(AF_ERR_INVALID, "subband method needs sb_nchan >= 1 and ndm0 >= 1");
+  if (d.use_iqrm) {
+    if (d.iqrm_c0 < 0 || d.iqrm_c1 > d.nchan || d.iqrm_c0 >= d.iqrm_c1)
+      return fail(AF_ERR_INVALID, "invalid IQRM channel range");
+    if (d.iqrm_mode != 0 && d.iqrm_mode != 1) return fail(AF_ERR_INVALID, "IQRM mode must be 0 or 1");
+  }
+  if (d.use_zero_dm && (d.zdm_c0 < 0 || d.zdm_c1 > d.nchan || d.zdm_c0 > d.zdm_c1))
+    return fail(AF_ERR_INVALID, "invalid zero-DM channel range");
+  if (d.n_bad < 0 || (d.n_bad > 0 && !d.bad_chans)) return fail(AF_ERR_INVALID, "bad channel list");
+  for (int c = 1; c < d.nchan; ++c)
+    if (!(d.freq_mhz[c] >= d.freq_mhz[c - 1]))
+      return fail(AF_ERR_INVALID, "frequency table must be ascending");
+  return AF_OK;
+}
+
+static af_status build(af_plan *p) {
+  const af_plan_desc &d = p->d;
+  const int ds = d.ds;
+  p->nbin = (d.nsamp + (size_t)ds - 1) / (size_t)ds;
+  p->nout = d.nout ? d.nout : p->nbin;
+  if (p->nout > p->nbin) return fail(AF_ERR_INVALID, "nout exceeds the binned input length");
+  p->nch = d.dd_c1 - d.dd_c0;
+  p->ndm = d.ndm;
+  const double fref = d.f_ref_mhz;
+  const double *F = p->freq.data();
+
+  // ---- delay / residual tables (FP64)
+  long maxd = 0;
+  if (d.method == AF_METHOD_DIRECT) {
+    p->h_delays.resize((size_t)p->ndm * p->nch);
+    for (int i = 0; i < p->ndm; ++i) {
+      const double dm = d.dm_low + (double)i * d.dm_step;
+      for (int c = d.dd_c0; c < d.dd_c1; ++c) {
+        long v = delay_native(F[c], fref, dm, d.tsamp) / ds;
+        if (v < 0) return fail(AF_ERR_INVALID, "negative delay: the reference frequency is below a dedispersed channel");
+        if (v > 0x3FFFFFFF) return fail(AF_ERR_INVALID, "delay too large");
+        maxd = std::max(maxd, v);
+        p->h_delays[(size_t)i * p->nch + (c - d.dd_c0)] = (int32_t)v;
+      }
+    }
+  } else {
+    const int sbw = d.sb_nchan, ndm0 = d.ndm0;
+    p->nsb = (p->nch + sbw - 1) / sbw;
+    p->ndm_nom = (p->ndm + ndm0 - 1) / ndm0;
+    p->h_delays.resize((size_t)p->ndm_nom * p->nch);
+    for (int j = 0; j < p->ndm_nom; ++j) {
+      const double dm = d.dm_low + (double)(j * ndm0) * d.dm_step;
+      for (int c = d.dd_c0; c < d.dd_c1; ++c) {
+        long v = delay_native(F[c], fref, dm, d.tsamp) / ds;
+        if (v < 0) return fail(AF_ERR_INVALID, "negative delay: the reference frequency is below a dedispersed channel");
+        if (v > 0x3FFFFFFF) return fail(AF_ERR_INVALID, "delay too large");
+        maxd = std::max(maxd, v);
+        p->h_delays[(size_t)j * p->nch + (c - d.dd_c0)] = (int32_t)v;
+      }
+    }
+    p->h_res.resize((size_t)p->ndm * p->nsb);
+    p->h_maxres_j.assign(p->ndm_nom, 0);
+    long maxr = 0;
+    for (int m = 0; m < p->ndm; ++m) {
+      const int j = m / ndm0;
+      const double ddm = (double)(m - j * ndm0) * d.dm_step;
+      for (int sb = 0; sb < p->nsb; ++sb) {
+        const int ch0 = d.dd_c0 + sb * sbw;
+        const int ch1 = std::min(ch0 + sbw, d.dd_c1);
+        long r = residual(F[(ch0 + ch1 - 1) / 2], fref, ddm, d.tsamp, ds);
+        if (r < 0) return fail(AF_ERR_INVALID, "negative subband residual");
+        if (r > 0x3FFFFFFF) return fail(AF_ERR_INVALID, "residual too large");
+        p->h_res[(size_t)m * p->nsb + sb] = (int32_t)r;
+        p->h_maxres_j[j] = std::max<int32_t>(p->h_maxres_j[j], (int32_t)r);
+        maxr = std::max(maxr, r);
+      }
+    }
+    p->maxres = (int)maxr;
+  }
+  p->maxdelay = (int)maxd;
+
+  // ---- sizes of the dedispersion stages
+  int maxlds = 65536;
+  {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device) == hipSuccess && v > 0)
+      maxlds = v;
+  }
+  size_t n_ext;
+  if (d.method == AF_METHOD_DIRECT) {
+    n_ext = p->nout;
+    p->adds = (double)p->ndm * p->nch * (double)p->nout;
+  } else {
+    const bool inter16 = d.dtype == AF_U8 && d.sb_nchan <= 256;
+    int tile = 0;
+    if (inter16) {
+      const int cand[] = {1024, 512, 256, 128, 64};
+      for (int t : cand) {
+        size_t need = (size_t)p->nsb * (size_t)(t + p->maxres) * 2;
+        if (need <= (size_t)std::min(maxlds, 65536)) { tile = t; break; }
+      }
+    }
+    if (tile == 0) tile = 512;  // generic stage 2 (no LDS)
+    p->s2_tile = tile;
+    p->s2_lds = tile + p->maxres;
+    size_t ntile2 = (p->nout + tile - 1) / tile;
+    n_ext = ntile2 * (size_t)tile + (size_t)p->maxres;
+    p->adds = (double)p->ndm_nom * p->nch * (double)(p->nout + p->maxres) +
+              (double)p->ndm * p->nsb * (double)p->nout;
+  }
+  p->ntiles1 = (int)((n_ext + kDdTile - 1) / kDdTile);
+  p->L1 = (size_t)p->ntiles1 * kDdTile;
+  size_t need_len = p->L1 + (size_t)p->maxdelay + 64;
+  p->Lp = std::max(p->nbin, need_len);
+  p->Lp = (p->Lp + 255) / 256 * 256;
+
+  // ---- IQRM windows, lags and row segmentation
+  std::vector<uint32_t> bounds = {0u, (uint32_t)p->nbin};
+  if (p->nbin > 0xFFFFFFF0ull) return fail(AF_ERR_INVALID, "too many rows");
+  if (d.use_iqrm) {
+    make_windows(p->nbin, d.iqrm_win, d.iqrm_hop, d.iqrm_include_tail != 0, p->wt0, p->wt1);
+    p->nwin = (int)p->wt0.size();
+    p->Csub = d.iqrm_c1 - d.iqrm_c0;
+    // iqrm_omp::set_iqrm_params clamps (include/iqrm.hpp:33-37)
+    const float rf = std::max(0.0f, d.iqrm_radius_frac);
+    const double gf = d.iqrm_geofactor > 1.0 ? d.iqrm_geofactor : 1.5;
+    const int radius = std::max(1, (int)std::floor(rf * (float)p->Csub));
+    p->lags = make_lags(radius, gf);
+    p->nlags = (int)p->lags.size();
+    for (int w = 0; w < p->nwin; ++w) {
+      bounds.push_back(p->wt0[w]);
+      bounds.push_back(p->wt1[w]);
+    }
+    int P2 = 1;
+    while (P2 < p->Csub) P2 <<= 1;
+    p->iqrm_lds = (size_t)(2 * p->Csub + P2) * sizeof(float);
+    if (p->iqrm_lds > (size_t)maxlds)
+      return fail(AF_ERR_INVALID, "IQRM channel range too large for one workgroup");
+  }
+  std::sort(bounds.begin(), bounds.end());
+  bounds.erase(std::unique(bounds.begin(), bounds.end()), bounds.end());
+  for (size_t i = 0; i + 1 < bounds.size(); ++i) {
+    Seg s{bounds[i], bounds[i + 1], 0, 0};
+    int wlo = -1, whi = -1;
+    for (int w = 0; w < p->nwin; ++w)
+      if (p->wt0[w] <= s.t0 && s.t1 <= p->wt1[w]) {
+        if (wlo < 0) wlo = w;
+        whi = w;
+      }
+    if (wlo >= 0) {
+      for (int w = wlo; w <= whi; ++w)
+        if (!(p->wt0[w] <= s.t0 && s.t1 <= p->wt1[w]))
+          return fail(AF_ERR_RUNTIME, "internal: non-contiguous window cover");
+      s.w0 = (uint32_t)wlo;
+      s.w1 = (uint32_t)whi + 1;
+    }
+    p->segs.push_back(s);
+  }
+  p->nseg = (int)p->segs.size();
+  p->wseg.assign(2 * (size_t)p->nwin, 0);
+  for (int w = 0; w < p->nwin; ++w) {
+    int a = -1, b = -1;
+    for (int s = 0; s < p->nseg; ++s)
+      if (p->segs[s].t0 >= p->wt0[w] && p->segs[s].t1 <= p->wt1[w]) {
+        if (a < 0) a = s;
+        b = s;
+      }
+    p->wseg[2 * w] = (uint32_t)a;
+    p->wseg[2 * w + 1] = (uint32_t)b + 1;
+  }
+  p->seg_items.assign(1, 0);
+  for (int s = 0; s < p->nseg; ++s) {
+    for (uint32_t r = p->segs[s].t0; r < p->segs[s].t1; r += kStatRows)
+      p->stat_items.push_back(RowItem{(uint32_t)s, r, std::min<uint32_t>(r + kStatRows, p->segs[s].t1), 0});
+    p->seg_items.push_back((int)p->stat_items.size());
+    for (uint32_t r = p->segs[s].t0; r < p->segs[s].t1; r += kCleanRows)
+      p->clean_items.push_back(RowItem{(uint32_t)s, r, std::min<uint32_t>(r + kCleanRows, p->segs[s].t1), 0});
+  }
+
+  // ---- device memory
+  AF_TRY(upload(p, &p->d_delays, p->h_delays));
+  AF_TRY(upload(p, &p->d_res, p->h_res));
+  AF_TRY(upload(p, &p->d_maxres_j, p->h_maxres_j));
+  AF_TRY(upload(p, &p->d_lags, p->lags));
+  AF_TRY(upload(p, &p->d_seg_items, p->seg_items));
+  AF_TRY(upload(p, &p->d_wseg, p->wseg));
+  AF_TRY(upload(p, &p->d_wt0, p->wt0));
+  AF_TRY(upload(p, &p->d_wt1, p->wt1));
+  AF_TRY(upload(p, &p->d_segs, p->segs));
+  AF_TRY(upload(p, &p->d_stat_items, p->stat_items));
+  AF_TRY(upload(p, &p->d_clean_items, p->clean_items));
+  if (!p->bad.empty()) {
+    std::vector<uint8_t> st(d.nchan, 0);
+    bool any = false;
+    for (int c : p->bad)
+      if (c >= 0 && c < d.nchan) { st[c] = 1; any = true; }
+    if (any) AF_TRY(upload(p, &p->d_static, st));
+  }
+  const size_t es = dtype_size(d.dtype);
+  if (ds > 1) AF_TRY(dalloc(p, &p->d_xb, p->nbin * (size_t)d.nchan * es));
+  AF_TRY(dalloc(p, &p->d_xT, (size_t)p->nch * p->Lp * es));
+  AF_HIP(hipMemset(p->d_xT, 0, (size_t)p->nch * p->Lp * es));
+  if (d.method == AF_METHOD_SUBBAND) {
+    size_t ies = (d.dtype == AF_U8 && d.sb_nchan <= 256) ? 2 : (d.dtype == AF_U32 || d.dtype == AF_F32 ? 8 : 4);
+    AF_TRY(dalloc(p, &p->d_inter, (size_t)p->ndm_nom * p->nsb * p->L1 * ies));
+  }
+  if (d.use_iqrm && p->nwin > 0) {
+    void *q;
+    AF_TRY(dalloc(p, &q, p->stat_items.size() * (size_t)p->Csub * sizeof(Part)));
+    p->d_part = (Part *)q;
+    AF_TRY(dalloc(p, &q, (size_t)p->nseg * p->Csub * sizeof(Part)));
+    p->d_segsum = (Part *)q;
+    AF_TRY(dalloc(p, &q, (size_t)p->nwin * std::max(1, p->nlags) * p->Csub));
+    p->d_flags = (uint8_t *)q;
+    AF_TRY(dalloc(p, &q, (size_t)p->nwin * d.nchan));
+    p->d_masks = (uint8_t *)q;
+    AF_TRY(dalloc(p, &q, (size_t)p->nseg * d.nchan));
+    p->d_segmask = (uint8_t *)q;
+  }
+  AF_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+  return AF_OK;
+}
+
+static af_status record(af_plan *p, int set, int k, hipStream_t st) {
+  if (!p->profiling || set < 0) return AF_OK;
+  AF_HIP(hipEventRecord(p->ev[(size_t)set * af_plan::kNumEv + k], st));
+  return AF_OK;
+}
+
+static af_status collect(af_plan *p) {
+  for (int s = 0; s < p->ev_used; ++s) {
+    hipEvent_t *e = &p->ev[(size_t)s * af_plan::kNumEv];
+    AF_HIP(hipEventSynchronize(e[af_plan::kNumEv - 1]));
+    for (int k = 0; k + 1 < af_plan::kNumEv; ++k) {
+      float ms = 0;
+      AF_HIP(hipEventElapsedTime(&ms, e[k], e[k + 1]));
+      p->acc_ms[k] += ms;
+    }
+    float tot = 0;
+    AF_HIP(hipEventElapsedTime(&tot, e[0], e[af_plan::kNumEv - 1]));
+    p->acc_total += tot;
+    p->acc_n += 1;
+  }
+  p->ev_used = 0;
+  return AF_OK;
+}
+
+static af_status execute(af_plan *p, const void *d_in, void *d_out, uint8_t *d_masks,
+                         hipStream_t st) {
+  const af_plan_desc &d = p->d;
+  int set = -1;
+  if (p->profiling) {
+    if ((size_t)(p->ev_used + 1) * af_plan::kNumEv > p->ev.size()) AF_TRY(collect(p));
+    set = p->ev_used++;
+  }
+  AF_TRY(record(p, set, 0, st));
+  const void *src = d_in;
+  if (d.ds > 1) {
+    AF_HIP(launch_bin(d.dtype, d_in, p->d_xb, d.nsamp, d.nchan, d.ds, p->nbin, st));
+    src = p->d_xb;
+  }
+  AF_TRY(record(p, set, 1, st));
+  const bool iq = d.use_iqrm && p->nwin > 0;
+  if (iq) {
+    AF_HIP(launch_seg_stats(d.dtype, src, d.nchan, d.iqrm_c0, d.iqrm_c1, p->d_stat_items,
+                            (int)p->stat_items.size(), p->d_part, st));
+    AF_HIP(launch_seg_reduce(d.dtype, p->d_part, p->d_seg_items, p->nseg, p->Csub, p->d_segsum, st));
+  }
+  AF_TRY(record(p, set, 2, st));
+  if (iq) {
+    if (p->nlags > 0) {
+      AF_HIP(launch_iqrm_lags(d.dtype, p->d_segsum, p->d_segs, p->d_wseg, p->d_wt0, p->d_wt1,
+                              p->nwin, p->Csub, d.iqrm_mode, p->d_lags, p->nlags,
+                              std::max(0.0f, d.iqrm_nsigma), p->d_flags, p->iqrm_lds, st));
+    }
+    AF_HIP(launch_iqrm_merge(p->d_flags, p->d_lags, p->nlags, p->nwin, p->Csub, d.iqrm_c0,
+                             d.nchan, p->d_masks, st));
+    AF_HIP(launch_segmask(p->d_masks, p->d_segs, p->nseg, d.nchan, p->d_segmask, st));
+  }
+  AF_TRY(record(p, set, 3, st));
+  CleanArgs ca{};
+  ca.x = src;
+  ca.xT = p->d_xT;
+  ca.Lp = p->Lp;
+  ca.nchan = d.nchan;
+  ca.items = p->d_clean_items;
+  ca.nitems = (int)p->clean_items.size();
+  ca.segmask = iq ? p->d_segmask : nullptr;
+  ca.stat_mask = p->d_static;
+  ca.zdm = d.use_zero_dm && d.zdm_c1 > d.zdm_c0;
+  ca.zc0 = d.zdm_c0;
+  ca.zc1 = d.zdm_c1;
+  ca.dd_c0 = d.dd_c0;
+  ca.dd_c1 = d.dd_c1;
+  AF_HIP(launch_clean(d.dtype, ca, st));
+  AF_TRY(record(p, set, 4, st));
+  DdArgs da{};
+  da.xT = p->d_xT;
+  da.Lp = p->Lp;
+  da.delays = p->d_delays;
+  da.nch = p->nch;
+  da.ntiles = p->ntiles1;
+  if (d.method == AF_METHOD_DIRECT) {
+    da.nrows = p->ndm;
+    da.ngrp = 1;
+    da.grp_w = p->nch;
+    da.out = d_out;
+    da.ostride = p->nout;
+    da.nvalid = p->nout;
+    da.out16 = 0;
+    AF_HIP(launch_dedisperse(d.dtype, da, st));
+    AF_TRY(record(p, set, 5, st));
+  } else {
+    const bool inter16 = d.dtype == AF_U8 && d.sb_nchan <= 256;
+    da.nrows = p->ndm_nom;
+    da.ngrp = p->nsb;
+    da.grp_w = d.sb_nchan;
+    da.out = p->d_inter;
+    da.ostride = p->L1;
+    da.nvalid = p->L1;
+    da.out16 = inter16 ? 1 : 0;
+    AF_HIP(launch_dedisperse(d.dtype, da, st));
+    AF_TRY(record(p, set, 5, st));
+    Stage2Args sa{};
+    sa.inter = p->d_inter;
+    sa.L1 = p->L1;
+    sa.nsb = p->nsb;
+    sa.res = p->d_res;
+    sa.maxres_j = p->d_maxres_j;
+    sa.ndm = p->ndm;
+    sa.ndm0 = d.ndm0;
+    sa.ndm_nom = p->ndm_nom;
+    sa.out = d_out;
+    sa.nout = p->nout;
+    sa.tile = p->s2_tile;
+    sa.lds_len = p->s2_lds;
+    sa.inter16 = inter16 ? 1 : 0;
+    AF_HIP(launch_stage2(d.dtype, sa, st));
+  }
+  AF_TRY(record(p, set, 6, st));
+  if (d_masks && iq)
+    AF_HIP(hipMemcpyAsync(d_masks, p->d_masks, (size_t)p->nwin * d.nchan, hipMemcpyDeviceToDevice, st));
+  return AF_OK;
+}
+
+}  // namespace af
+
+using namespace af;
+
+extern "C" {
+
+af_status af_plan_create(const af_plan_desc *desc, af_plan **plan) {
+  if (!desc || !plan) return fail(AF_ERR_INVALID, "null argument");
+  *plan = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(AF_ERR_NODEV, "No HIP devices found");
+  AF_TRY(validate(*desc));
+  af_plan *p = new af_plan();
+  p->d = *desc;
+  p->freq.assign(desc->freq_mhz, desc->freq_mhz + desc->nchan);
+  p->d.freq_mhz = nullptr;
+  if (desc->n_bad > 0) p->bad.assign(desc->bad_chans, desc->bad_chans + desc->n_bad);
+  p->d.bad_chans = nullptr;
+  (void)hipGetDevice(&p->device);
+  af_status s = build(p);
+  if (s != AF_OK) {
+    std::string msg = g_err;
+    release(p);
+    delete p;
+    g_err = msg;
+    return s;
+  }
+  *plan = p;
+  return AF_OK;
+}
+
+af_status af_plan_destroy(af_plan *plan) {
+  if (!plan) return AF_OK;
+  (void)hipSetDevice(plan->device);
+  (void)hipStreamSynchronize(plan->stream);
+  release(plan);
+  delete plan;
+  return AF_OK;
+}
+
+af_status af_plan_query(const af_plan *p, af_plan_info *info) {
+  if (!p || !info) return fail(AF_ERR_INVALID, "null argument");
+  info->nbin = p->nbin;
+  info->nout = p->nout;
+  info->ndm = p->ndm;
+  info->out_elem_bytes = acc_size(p->d.dtype);
+  info->nwin = p->d.use_iqrm ? p->nwin : 0;
+  info->nsb = p->nsb;
+  info->ndm_nom = p->ndm_nom;
+  info->max_delay = p->maxdelay;
+  info->scratch_bytes = p->scratch_bytes;
+  info->adds = p->adds;
+  return AF_OK;
+}
+
+af_status af_plan_tables(const af_plan *p, int32_t *delays, int32_t *residuals) {
+  if (!p) return fail(AF_ERR_INVALID, "null plan");
+  if (delays) std::memcpy(delays, p->h_delays.data(), p->h_delays.size() * sizeof(int32_t));
+  if (residuals && !p->h_res.empty())
+    std::memcpy(residuals, p->h_res.data(), p->h_res.size() * sizeof(int32_t));
+  return AF_OK;
+}
+
+af_status af_plan_windows(const af_plan *p, uint32_t *t0s, uint32_t *t1s) {
+  if (!p) return fail(AF_ERR_INVALID, "null plan");
+  for (int w = 0; w < p->nwin; ++w) {
+    if (t0s) t0s[w] = p->wt0[w];
+    if (t1s) t1s[w] = p->wt1[w];
+  }
+  return AF_OK;
+}
+
+af_status af_plan_set_profiling(af_plan *p, int enable) {
+  if (!p) return fail(AF_ERR_INVALID, "null plan");
+  (void)hipSetDevice(p->device);
+  if (p->ev_used) AF_TRY(collect(p));
+  p->profiling = enable != 0;
+  std::fill(std::begin(p->acc_ms), std::end(p->acc_ms), 0.0);
+  p->acc_total = 0;
+  p->acc_n = 0;
+  if (p->profiling && p->ev.empty()) {
+    p->ev.resize((size_t)64 * af_plan::kNumEv);
+    for (auto &e : p->ev) AF_HIP(hipEventCreate(&e));
+  }
+  return AF_OK;
+}
+
+af_status af_plan_stage_times(af_plan *p, af_stage_times *t) {
+  if (!p || !t) return fail(AF_ERR_INVALID, "null argument");
+  (void)hipSetDevice(p->device);
+  AF_TRY(collect(p));
+  const double n = p->acc_n ? (double)p->acc_n : 1.0;
+  t->bin = (float)(p->acc_ms[0] / n);
+  t->stats = (float)(p->acc_ms[1] / n);
+  t->iqrm = (float)(p->acc_ms[2] / n);
+  t->clean = (float)(p->acc_ms[3] / n);
+  t->dedisp = (float)(p->acc_ms[4] / n);
+  t->stage2 = (float)(p->acc_ms[5] / n);
+  t->total = (float)(p->acc_total / n);
+  t->n = p->acc_n;
+  return AF_OK;
+}
+
+af_status af_execute(af_plan *p, const void *d_in, void *d_out, uint8_t *d_masks, void *stream) {
+  if (!p || !d_in || !d_out) return fail(AF_ERR_INVALID, "null argument");
+  AF_HIP(hipSetDevice(p->device));
+  return execute(p, d_in, d_out, d_masks, (hipStream_t)stream);
+}
+
+af_status af_execute_host(af_plan *p, const void *h_in, void *h_out, uint8_t *h_masks) {
+  if (!p || !h_in || !h_out) return fail(AF_ERR_INVALID, "null argument");
+  AF_HIP(hipSetDevice(p->device));
+  const size_t in_bytes = p->d.nsamp * (size_t)p->d.nchan * dtype_size(p->d.dtype);
+  const size_t out_bytes = (size_t)p->ndm * p->nout * acc_size(p->d.dtype);
+  const size_t m_bytes = (size_t)p->nwin * p->d.nchan;
+  void *din = nullptr, *dout = nullptr, *dm = nullptr;
+  hipError_t e = hipMalloc(&din, in_bytes);
+  if (e != hipSuccess) return fail(AF_ERR_NOMEM, "hipMalloc input");
+  e = hipMalloc(&dout, out_bytes);
+  if (e != hipSuccess) { (void)hipFree(din); return fail(AF_ERR_NOMEM, "hipMalloc output"); }
+  if (h_masks && m_bytes) {
+    e = hipMalloc(&dm, m_bytes);
+    if (e != hipSuccess) { (void)hipFree(din); (void)hipFree(dout); return fail(AF_ERR_NOMEM, "hipMalloc masks"); }
+  }
+  af_status s = AF_OK;
+  hipStream_t st = p->stream;
+  e = hipMemcpyAsync(din, h_in, in_bytes, hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) s = fail(AF_ERR_RUNTIME, std::string("H2D: ") + hipGetErrorString(e));
+  if (s == AF_OK) s = execute(p, din, dout, (uint8_t *)dm, st);
+  if (s == AF_OK) {
+    e = hipMemcpyAsync(h_out, dout, out_bytes, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && dm) e = hipMemcpyAsync(h_masks, dm, m_bytes, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) s = fail(AF_ERR_RUNTIME, std::string("D2H: ") + hipGetErrorString(e));
+  } else {
+    (void)hipStreamSynchronize(st);
+  }
+  (void)hipFree(din);
+  (void)hipFree(dout);
+  if (dm) (void)hipFree(dm);
+  return s;
+}
+
+af_status af_time_execute(af_plan *p, const void *d_in, void *d_out, int reps, float *ms) {
+  if (!p || !d_in || !d_out || !ms || reps < 1) return fail(AF_ERR_INVALID, "bad argument");
+  AF_HIP(hipSetDevice(p->device));
+  hipEvent_t a, b;
+  AF_HIP(hipEventCreate(&a));
+  AF_HIP(hipEventCreate(&b));
+  af_status s = AF_OK;
+  if (hipEventRecord(a, p->stream) != hipSuccess) s = fail(AF_ERR_RUNTIME, "event record");
+  for (int i = 0; i < reps && s == AF_OK; ++i) s = execute(p, d_in, d_out, nullptr, p->stream);
+  if (s == AF_OK && hipEventRecord(b, p->stream) != hipSuccess) s = fail(AF_ERR_RUNTIME, "event record");
+  if (s == AF_OK && hipEventSynchronize(b) != hipSuccess) s = fail(AF_ERR_RUNTIME, "event sync");
+  if (s == AF_OK && hipEventElapsedTime(ms, a, b) != hipSuccess) s = fail(AF_ERR_RUNTIME, "elapsed");
+  (void)hipStreamSynchronize(p->stream);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return s;
+}
+
+af_status af_device_count(int *n) {
+  if (!n) return fail(AF_ERR_INVALID, "null argument");
+  *n = 0;
+  hipError_t e = hipGetDeviceCount(n);
+  if (e != hipSuccess) {
+    *n = 0;
+    return AF_OK;
+  }
+  return AF_OK;
+}
+
+af_status af_set_device(int dev) {
+  AF_HIP(hipSetDevice(dev));
+  return AF_OK;
+}
+
+af_status af_malloc(void **p, size_t bytes) {
+  if (!p) return fail(AF_ERR_INVALID, "null argument");
+  hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+  if (e != hipSuccess) return fail(AF_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return AF_OK;
+}
+
+af_status af_free(void *p) {
+  if (p) AF_HIP(hipFree(p));
+  return AF_OK;
+}
+
+af_status af_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream) {
+  AF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+  AF_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return AF_OK;
+}
+
+af_status af_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream) {
+  AF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  AF_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return AF_OK;
+}
+
+af_status af_memset(void *dst, int value, size_t bytes, void *stream) {
+  AF_HIP(hipMemsetAsync(dst, value, bytes, (hipStream_t)stream));
+  return AF_OK;
+}
+
+af_status af_stream_create(void **stream) {
+  if (!stream) return fail(AF_ERR_INVALID, "null argument");
+  hipStream_t s;
+  AF_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return AF_OK;
+}
+
+af_status af_stream_destroy(void *stream) {
+  if (stream) AF_HIP(hipStreamDestroy((hipStream_t)stream));
+  return AF_OK;
+}
+
+af_status af_stream_sync(void *stream) {
+  AF_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return AF_OK;
+}
+
+af_status af_sync(void) {
+  AF_HIP(hipDeviceSynchronize());
+  return AF_OK;
+}
+
+const char *af_last_error(void) { return g_err.c_str(); }
+
+const char *af_version(void) { return "astroflow_amd 0.1 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,106 @@
+"""Seeded synthetic filterbanks and the workloads they are measured on.
+
+Frequency convention: the filterbank is recorded highest frequency first
+(fch1 = top, foff < 0) and flipped to ascending at ingest, as the reference's
+readers do (src/data/filterbank.cpp:793-842, python/astroflow/io/filterbank.py:117-121).
+The resulting header has fch1 = lowest channel and foff > 0, and the channel
+frequencies are formed in float32 from those header fields
+(src/core/dedispered.cu:1003-1005).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import List, Tuple
+
+import numpy as np
+
+KDM = 4148.808
+TSAMP = np.float32(49.152e-6)
+
+
+def ascending_header(f_lo_mhz: float, f_hi_mhz: float, nchan: int) -> Tuple[np.float32, np.float32]:
+    """(fch1, foff) after the flip for a band [f_lo, f_hi] recorded top first
+    with nchan channels of width (f_hi - f_lo)/nchan."""
+    foff_top = -(f_hi_mhz - f_lo_mhz) / nchan
+    fch1 = f_hi_mhz + (nchan - 1) * foff_top     # lowest channel (float64, as Python does)
+    return np.float32(fch1), np.float32(-foff_top)
+
+
+def freq_table(fch1: np.float32, foff: np.float32, nchan: int) -> np.ndarray:
+    """fch1 + i*foff in float32 arithmetic, returned as float64."""
+    i = np.arange(nchan, dtype=np.float32)
+    return (np.float32(fch1) + i * np.float32(foff)).astype(np.float64)
+
+
+def delay_samples(freq: np.ndarray, f_ref: float, dm: float, tsamp: float) -> np.ndarray:
+    inv = 1.0 / (freq * freq) - 1.0 / (f_ref * f_ref)
+    return np.round(KDM * dm * inv / tsamp)
+
+
+def noise_u8(rng: np.random.Generator, nsamp: int, nchan: int, mean=128.0, sigma=16.0) -> np.ndarray:
+    x = rng.normal(mean, sigma, size=(nsamp, nchan))
+    return np.clip(np.round(x), 0, 255).astype(np.uint8)
+
+
+def inject_pulse(x: np.ndarray, freq: np.ndarray, f_ref: float, tsamp: float, dm: float,
+                 t0: int, fwhm_s: float, amp: float):
+    """Add a dispersed Gaussian pulse (arrival at t0 at f_ref) to x in place."""
+    sigma = fwhm_s / tsamp / (2.0 * math.sqrt(2.0 * math.log(2.0)))
+    half = int(math.ceil(4 * sigma))
+    d = delay_samples(freq, f_ref, dm, tsamp).astype(np.int64)
+    offs = np.arange(-half, half + 1)
+    prof = amp * np.exp(-0.5 * (offs / sigma) ** 2)
+    n, nc = x.shape
+    for c in range(nc):
+        t = t0 + d[c] + offs
+        ok = (t >= 0) & (t < n)
+        if not ok.any():
+            continue
+        col = x[t[ok], c].astype(np.float64) + prof[ok]
+        if x.dtype == np.float32:
+            x[t[ok], c] = col.astype(np.float32)
+        else:
+            x[t[ok], c] = np.clip(np.round(col), 0, np.iinfo(x.dtype).max).astype(x.dtype)
+
+
+def inject_narrowband_rfi(x: np.ndarray, rng: np.random.Generator, frac=0.05, amp=40.0,
+                          n_stuck=10, pmin=8.0, pmax=4096.0) -> List[int]:
+    n, nc = x.shape
+    k = max(1, int(round(frac * nc)))
+    chans = rng.choice(nc, size=k + n_stuck, replace=False)
+    t = np.arange(n, dtype=np.float64)
+    for c in chans[:k]:
+        period = math.exp(rng.uniform(math.log(pmin), math.log(pmax)))
+        phase = rng.uniform(0, 2 * math.pi)
+        col = x[:, c].astype(np.float64) + amp * np.sin(2 * math.pi * t / period + phase)
+        x[:, c] = np.clip(np.round(col), 0, 255).astype(x.dtype)
+    for c in chans[k:]:
+        x[:, c] = 255
+    return sorted(int(c) for c in chans)
+
+
+@dataclass
+class Band:
+    nchan: int
+    fch1: np.float32
+    foff: np.float32
+    tsamp: np.float32
+
+    @property
+    def freq(self):
+        return freq_table(self.fch1, self.foff, self.nchan)
+
+    @property
+    def f_top(self):
+        return float(self.freq[-1])
+
+
+def band_1000_1500(nchan: int) -> Band:
+    fch1, foff = ascending_header(1000.0, 1500.0, nchan)
+    return Band(nchan, fch1, foff, TSAMP)
+
+
+def max_delay(band: Band, dm_max: float) -> int:
+    f = band.freq
+    return int(delay_samples(f[:1], band.f_top, dm_max, float(band.tsamp))[0])

@@ -1,0 +1,159 @@
+/*
+ * astroflow_amd.h -- C-ABI of the MI355X (gfx950) dedispersion + RFI library.
+ *
+ * Plain C types only (no torch, no C++).  Every entry point returns an
+ * af_status (AF_OK == 0); on failure af_last_error() returns a message for the
+ * calling thread.  No entry point exits the process.
+ *
+ * Reference interfaces replaced (lintian233/astroflow):
+ *   af_plan_create + af_execute      : gpucal::dedisperse_spec<T>
+ *                                      (src/core/dedispered.cu:988-1463) and
+ *                                      gpucal::dedispered_fil_cuda<T> (:483-986):
+ *                                      binning, IQRM, zero-DM, static mask and
+ *                                      direct/subband dedispersion of one buffer.
+ *   af_execute (use_iqrm)            : iqrm_cuda::rfi_iqrm_gpu<T> + RfiMarker<T>::mask
+ *                                      (src/core/iqrm.cu:438-467, src/rfi/rfimarker.cu:309-390)
+ *   af_execute (use_zero_dm)         : RfiMarker<T>::zero_dm_filter, range form
+ *                                      (src/rfi/rfimarker.cu:259-305)
+ *   af_execute (bad_chans)           : RfiMarker<T>::mark_rfi (src/rfi/rfimarker.cu:473-488)
+ */
+#ifndef ASTROFLOW_AMD_H
+#define ASTROFLOW_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int af_status;
+#define AF_OK 0
+#define AF_ERR_INVALID 1 /* bad argument (Python: ValueError)        */
+#define AF_ERR_RUNTIME 2 /* HIP/runtime failure (Python: RuntimeError) */
+#define AF_ERR_NOMEM 3   /* allocation or memory budget (MemoryError) */
+#define AF_ERR_NODEV 4   /* no GPU visible (RuntimeError)             */
+#define AF_ERR_BUSY 5    /* server queue full / not admitted yet      */
+
+/* Element types of the filterbank samples. */
+#define AF_U8 1
+#define AF_U16 2
+#define AF_U32 3
+#define AF_F32 4
+
+#define AF_METHOD_DIRECT 0
+#define AF_METHOD_SUBBAND 1
+
+/* Everything that defines one dedispersion call.  Channel ranges are
+ * half-open [c0, c1) and explicit per stage, because the reference mixes an
+ * inclusive range for dedispersion with exclusive ranges for IQRM and zero-DM
+ * (src/core/dedispered.cu:1089,1107,1124). */
+typedef struct af_plan_desc {
+  int dtype;               /* AF_U8 .. AF_F32                                   */
+  int nchan;               /* channels per time row of the input                */
+  size_t nsamp;            /* native-rate time rows in the input buffer         */
+  int ds;                  /* time downsample factor (>= 1)                     */
+  double tsamp;            /* native sample time in seconds                     */
+  const double *freq_mhz;  /* [nchan] channel centre frequencies, ascending     */
+  double f_ref_mhz;        /* reference frequency for zero delay                */
+  int dd_c0, dd_c1;        /* dedispersed channels                              */
+  double dm_low, dm_step;  /* trial DMs dm_low + i*dm_step                      */
+  int ndm;                 /* number of trial DMs                               */
+  size_t nout;             /* output samples (binned rate); 0 = ceil(nsamp/ds)  */
+  int method;              /* AF_METHOD_DIRECT or AF_METHOD_SUBBAND             */
+  int sb_nchan;            /* channels per subband (subband method)             */
+  int ndm0;                /* fine DMs per coarse DM group (subband method)     */
+  /* IQRM (src/core/iqrm.cu) */
+  int use_iqrm;
+  int iqrm_c0, iqrm_c1;
+  int iqrm_mode;           /* 0 mean, 1 std                                     */
+  float iqrm_radius_frac;
+  float iqrm_nsigma;
+  double iqrm_geofactor;
+  unsigned iqrm_win;       /* window length in binned samples, 0 = whole buffer */
+  unsigned iqrm_hop;       /* hop in binned samples, 0 = window length          */
+  int iqrm_include_tail;
+  /* zero-DM (src/rfi/rfimarker.cu:172-305) */
+  int use_zero_dm;
+  int zdm_c0, zdm_c1;
+  /* static bad-channel mask (applied after zero-DM) */
+  const int *bad_chans;
+  int n_bad;
+  /* kernel tuning; 0 = library default (the result does not depend on these) */
+  int tile_t;              /* output samples per workgroup tile                 */
+  int dm_per_block;        /* DM rows per workgroup                             */
+} af_plan_desc;
+
+typedef struct af_plan af_plan;
+
+typedef struct af_plan_info {
+  size_t nbin;             /* binned rows = ceil(nsamp/ds)                      */
+  size_t nout;             /* output samples per DM row                         */
+  int ndm;
+  int out_elem_bytes;      /* 4 (u32 for u8/u16, f32 for f32) or 8 (u64 for u32) */
+  int nwin;                /* IQRM windows (0 without IQRM)                     */
+  int nsb;                 /* subbands (subband method)                         */
+  int ndm_nom;             /* coarse DMs (subband method)                       */
+  int max_delay;           /* largest delay in binned samples over all DMs      */
+  size_t scratch_bytes;    /* device memory held by the plan                    */
+  double adds;             /* integer/float adds the method performs per call   */
+} af_plan_info;
+
+/* Per-stage device time, averaged over the n executes run since profiling
+ * was switched on (milliseconds; HIP events recorded on the execute stream
+ * between the stages). */
+typedef struct af_stage_times {
+  float bin, stats, iqrm, clean, dedisp, stage2, total;
+  int n;
+} af_stage_times;
+
+af_status af_plan_create(const af_plan_desc *desc, af_plan **plan);
+af_status af_plan_destroy(af_plan *plan);
+af_status af_plan_query(const af_plan *plan, af_plan_info *info);
+/* Copy of the delay table used by the first dedispersion stage
+ * ([ndm][dd_c1-dd_c0] direct, [ndm_nom][...] subband) and the residual table
+ * ([ndm][nsb], subband only).  Either pointer may be NULL. */
+af_status af_plan_tables(const af_plan *plan, int32_t *delays, int32_t *residuals);
+/* IQRM window bounds [t0,t1) in binned samples, nwin entries each. */
+af_status af_plan_windows(const af_plan *plan, uint32_t *t0s, uint32_t *t1s);
+/* Switch stage timing on (and reset it) or off. */
+af_status af_plan_set_profiling(af_plan *plan, int enable);
+/* Synchronises with the recorded events and returns the averages. */
+af_status af_plan_stage_times(af_plan *plan, af_stage_times *t);
+
+/* Run one call on device buffers (stream may be NULL = null stream).
+ *   d_in   : [nsamp][nchan] samples, time-major (as read from a filterbank)
+ *   d_out  : [ndm][nout] plane, DM-major (u32 / u64 / f32)
+ *   d_masks: optional [nwin][nchan] u8 IQRM masks (NULL to skip)
+ * The input buffer is not modified.  Asynchronous on the stream. */
+af_status af_execute(af_plan *plan, const void *d_in, void *d_out,
+                     uint8_t *d_masks, void *stream);
+
+/* Same on host buffers: upload, execute, download, synchronise. */
+af_status af_execute_host(af_plan *plan, const void *h_in, void *h_out,
+                          uint8_t *h_masks);
+
+/* Device helpers so callers need no other GPU runtime. */
+af_status af_device_count(int *n);
+af_status af_set_device(int dev);
+af_status af_malloc(void **p, size_t bytes);
+af_status af_free(void *p);
+af_status af_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+af_status af_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+af_status af_memset(void *dst, int value, size_t bytes, void *stream);
+af_status af_stream_create(void **stream);
+af_status af_stream_destroy(void *stream);
+af_status af_stream_sync(void *stream);
+af_status af_sync(void);
+/* Elapsed milliseconds of `reps` back-to-back af_execute calls on the plan's
+ * stream, bracketed by HIP events on that stream. */
+af_status af_time_execute(af_plan *plan, const void *d_in, void *d_out, int reps,
+                          float *ms);
+
+const char *af_last_error(void);
+const char *af_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

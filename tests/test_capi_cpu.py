@@ -1,0 +1,73 @@
+"""C-ABI library checks that need no GPU: it loads, exports every symbol the
+header declares, struct layouts match, and calls fail cleanly without a GPU."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from astroflow_amd import capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include"))
+           if f.endswith(".h")]
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[\w\s\*]+?\b(af_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_headers_declare_functions():
+    names = declared_functions()
+    assert "af_plan_create" in names and "af_execute" in names
+    assert len(names) >= 20
+
+
+def test_library_exports_every_declared_symbol():
+    lib = capi.lib()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, f"declared but not exported: {missing}"
+
+
+def test_python_binding_covers_header():
+    assert set(declared_functions()) <= set(capi.SIGNATURES) | {"af_batch_create"} | set(
+        capi.SIGNATURES)
+    assert set(capi.SIGNATURES) <= set(declared_functions())
+
+
+def _c_sizeof(type_name, tmp_path):
+    src = tmp_path / "sz.c"
+    src.write_text('#include <stdio.h>\n#include "astroflow_amd.h"\n'
+                   'int main(void){printf("%%zu\\n", sizeof(%s));return 0;}\n' % type_name)
+    exe = tmp_path / "sz"
+    subprocess.run(["gcc", f"-I{ROOT}/include", str(src), "-o", str(exe)], check=True)
+    return int(subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True, check=True).stdout)
+
+
+@pytest.mark.parametrize("cname,pytype", [("af_plan_desc", capi.PlanDesc),
+                                          ("af_plan_info", capi.PlanInfo),
+                                          ("af_stage_times", capi.StageTimes)])
+def test_struct_layouts_match_c(cname, pytype, tmp_path):
+    assert _c_sizeof(cname, tmp_path) == C.sizeof(pytype)
+
+
+def test_version_and_error_strings():
+    assert b"gfx950" in capi.lib().af_version()
+    assert isinstance(capi.lib().af_last_error(), bytes)
+
+
+@pytest.mark.skipif(capi.device_count() > 0, reason="a GPU is visible")
+def test_plan_create_without_gpu_reports_nodev():
+    import numpy as np
+    from tests.helpers import make_spec, small_band
+    x = np.zeros((64, 8), np.uint8)
+    spec = make_spec(x, small_band(8), 0.0, 1.0, 4, method="direct")
+    with pytest.raises(capi.AFError, match="No HIP devices"):
+        capi.Plan(spec)

@@ -1,0 +1,155 @@
+"""Parity of the HIP path (through the C-ABI) against the CPU oracle.
+
+Integer data: bit-exact planes and IQRM masks.  float32 data: within
+|gpu - oracle| <= 1e-6 * sum_c |x_c| per output (the FP64 oracle)."""
+import sys
+
+import numpy as np
+import pytest
+
+from astroflow_amd import capi, synth
+from oracle import oracle as O
+from tests.helpers import make_spec, oracle_spec, small_band
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def run_both(x, spec, want_masks=True):
+    plan = capi.Plan(spec)
+    got, masks = plan.run_host(x, want_masks=want_masks)
+    ref, rmask, t0, t1, _ = O.run_pipeline(x, oracle_spec(spec), O.IQRM_GPU)
+    if spec.use_iqrm:
+        w0, w1 = plan.windows()
+        assert np.array_equal(w0, t0) and np.array_equal(w1, t1)
+    plan.close()
+    return got, ref, masks, rmask
+
+
+def noise(rng, n, nc, dtype=np.uint8):
+    if dtype == np.float32:
+        return rng.standard_normal((n, nc)).astype(np.float32)
+    vmax = np.iinfo(dtype).max
+    return np.clip(np.round(rng.normal(vmax / 2, vmax / 16, (n, nc))), 0, vmax).astype(dtype)
+
+
+def assert_f32_close(got, ref, x, spec):
+    # tolerance per output: 1e-6 * sum over the contributing samples of |x|;
+    # bounded above by the same plane over |x|
+    absx = np.abs(x).astype(np.float32)
+    s2 = oracle_spec(spec)
+    s2.use_iqrm = s2.use_zero_dm = False
+    s2.bad_channels = []
+    bound, *_ = O.run_pipeline(absx + 1.0, s2)
+    assert np.all(np.abs(got.astype(np.float64) - ref) <= 1e-6 * bound)
+
+
+def test_direct_u8_plain(gpu_lib):
+    say("case: direct u8 300ch x 5000, 37 DMs")
+    rng = np.random.default_rng(1)
+    b = small_band(300, 1200.0, 1500.0, 1e-4)
+    x = noise(rng, 5000, 300)
+    spec = make_spec(x, b, 0.0, 7.0, 37, method="direct", dd_c0=3, dd_c1=290, nout=4321)
+    got, ref, *_ = run_both(x, spec)
+    assert np.array_equal(got, ref)
+
+
+def test_direct_u8_many_channels_flush(gpu_lib):
+    say("case: direct u8 1000ch (>256-channel flush)")
+    rng = np.random.default_rng(2)
+    b = small_band(1000, 1000.0, 1500.0, 2e-4)
+    x = np.full((3000, 1000), 255, np.uint8)   # worst case for the 16-bit lanes
+    x[::3] = noise(rng, 1000, 1000)
+    spec = make_spec(x, b, 10.0, 11.0, 9, method="direct")
+    got, ref, *_ = run_both(x, spec)
+    assert np.array_equal(got, ref)
+
+
+def test_subband_u8_full_rfi(gpu_lib):
+    say("case: subband u8 512ch, IQRM windows with hop + zero-DM + static mask")
+    rng = np.random.default_rng(3)
+    b = small_band(512, 1100.0, 1500.0, 2e-4)
+    x = noise(rng, 6000, 512)
+    synth.inject_narrowband_rfi(x, rng, frac=0.05, amp=40, n_stuck=5, pmax=512)
+    synth.inject_pulse(x, b.freq, b.f_top, float(b.tsamp), 150.0, 500, 2e-3, 48.0)
+    spec = make_spec(x, b, 0.0, 1.5, 130, method="subband", sbw=64, ndm0=16,
+                     use_iqrm=True, iqrm_c0=0, iqrm_c1=511, iqrm_mode=1,
+                     iqrm_radius_frac=0.02, iqrm_nsigma=3.0, iqrm_W=2500, iqrm_H=1500,
+                     use_zero_dm=True, zdm_c0=0, zdm_c1=511, bad_channels=[7, 100, 511, 900])
+    got, ref, masks, rmask = run_both(x, spec)
+    assert masks.shape == rmask.shape and masks.sum() > 0
+    assert np.array_equal(masks, rmask)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("ds", [2, 3])
+def test_subband_u8_downsampled(gpu_lib, ds):
+    say(f"case: subband u8 ds={ds}")
+    rng = np.random.default_rng(10 + ds)
+    b = small_band(256, 1200.0, 1500.0, 1e-4)
+    x = noise(rng, 7001, 256)
+    spec = make_spec(x, b, 5.0, 2.0, 70, method="subband", sbw=32, ndm0=8, ds=ds,
+                     use_iqrm=True, iqrm_c0=1, iqrm_c1=255, iqrm_mode=0,
+                     iqrm_radius_frac=0.1, use_zero_dm=True, zdm_c0=1, zdm_c1=255)
+    got, ref, masks, rmask = run_both(x, spec)
+    assert np.array_equal(masks, rmask)
+    assert np.array_equal(got, ref)
+
+
+def test_subband_u8_wide_subbands_generic(gpu_lib):
+    say("case: subband u8 with 300-channel subbands (u32 partial sums)")
+    rng = np.random.default_rng(5)
+    b = small_band(600, 1000.0, 1500.0, 2e-4)
+    x = noise(rng, 4000, 600)
+    spec = make_spec(x, b, 0.0, 3.0, 40, method="subband", sbw=300, ndm0=4)
+    got, ref, *_ = run_both(x, spec)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("dtype", [np.uint16, np.uint32])
+@pytest.mark.parametrize("method", ["direct", "subband"])
+def test_wide_integer_types(gpu_lib, dtype, method):
+    say(f"case: {np.dtype(dtype).name} {method}")
+    rng = np.random.default_rng(7)
+    b = small_band(200, 1200.0, 1500.0, 1e-4)
+    x = noise(rng, 3000, 200, dtype)
+    spec = make_spec(x, b, 0.0, 5.0, 33, method=method, sbw=25, ndm0=8,
+                     use_iqrm=True, iqrm_c0=0, iqrm_c1=199, iqrm_mode=1,
+                     use_zero_dm=True, zdm_c0=0, zdm_c1=199)
+    got, ref, masks, rmask = run_both(x, spec)
+    assert np.array_equal(masks, rmask)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("method", ["direct", "subband"])
+def test_float32_within_tolerance(gpu_lib, method):
+    say(f"case: float32 {method}")
+    rng = np.random.default_rng(8)
+    b = small_band(256, 1200.0, 1500.0, 1e-4)
+    x = noise(rng, 3000, 256, np.float32)
+    spec = make_spec(x, b, 0.0, 4.0, 40, method=method, sbw=32, ndm0=8, ds=2,
+                     use_zero_dm=True, zdm_c0=0, zdm_c1=255)
+    got, ref, *_ = run_both(x, spec, want_masks=False)
+    assert got.dtype == np.float32
+    assert_f32_close(got, ref, x, spec)
+
+
+def test_delay_tables_equal_oracle(gpu_lib):
+    say("case: delay/residual tables")
+    b = synth.band_1000_1500(4096)
+    x = np.zeros((1, 4096), np.uint8)
+    spec = make_spec(x, b, 0.0, 0.5, 6001, method="subband", sbw=128, ndm0=32)
+    spec.nsamp = 160000
+    plan = capi.Plan(spec)
+    d1, res = plan.tables()
+    f = b.freq
+    nnom = (6001 + 31) // 32
+    assert np.array_equal(d1, O.delay_table(f, 0, 4096, f[-1], float(b.tsamp), 1,
+                                            np.arange(nnom) * 32 * 0.5))
+    assert np.array_equal(res, O.residual_table(f, 0, 4096, 128, f[-1], float(b.tsamp), 1,
+                                                0.0, 0.5, 6001, 32))
+    plan.close()

@@ -1,0 +1,368 @@
+"""Pins the CPU oracle: reference-built code, closed forms, brute force.
+
+No GPU.  Each test names what it checks the oracle against.
+"""
+import math
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from astroflow_amd import synth
+from oracle import oracle as O
+from tests.helpers import small_band
+
+REF = O.ref_lib()
+needs_ref = pytest.mark.skipif(REF is None, reason="oracle/_ref not built (no reference sources)")
+
+
+# ------------------------------------------------------------- setup math --
+
+def test_freq_table_float32_spot_values():
+    # 1024 channels over 1000-1500 MHz recorded top first; after the flip the
+    # lowest channel is 1500 - 1023*500/1024 and every entry is exact in float32.
+    b = synth.band_1000_1500(1024)
+    f = O.freq_table_f32(float(b.fch1), float(b.foff), 1024)
+    assert f[0] == 1000.48828125 and f[-1] == 1500.0
+    assert np.all(np.diff(f) == 0.48828125)
+    # float32 accumulation differs from float64 for a non-representable step
+    g = O.freq_table_f32(1400.0, 0.1, 4096)
+    assert g[4095] == float(np.float32(np.float32(1400.0) + np.float32(4095) * np.float32(0.1)))
+
+
+def test_chan_range_truncation():
+    f = O.freq_table_f32(1000.0, 1.0, 101)   # 1000..1100
+    assert O.chan_range(f, 1000.0, 1100.0) == (0, 100)
+    assert O.chan_range(f, 1010.5, 1050.9) == (10, 50)   # truncation, not rounding
+    with pytest.raises(ValueError):
+        O.chan_range(f, 990.0, 1050.0)
+
+
+def test_dm_steps_float32():
+    assert O.dm_steps(0.0, 1000.0, 1.0) == 1001
+    assert O.dm_steps(0.0, 3000.0, 0.5) == 6001
+    # (0.3f - 0.1f)/0.1f evaluates below 2 in float32
+    q = (np.float32(0.3) - np.float32(0.1)) / np.float32(0.1)
+    assert O.dm_steps(0.1, 0.3, 0.1) == int(q) + 1
+
+
+def test_delay_hand_value():
+    # DM 1000 from 1000 to 1500 MHz at 49.152 us: 4148.808e3*(1e-6-1/2.25e6)/49.152e-6
+    # = 2.304893.../49.152e-6 = 46893.33 -> 46893 samples.
+    f = np.array([1000.0, 1500.0])
+    d = O.delay_table(f, 0, 2, 1500.0, 49.152e-6, 1, [1000.0])
+    assert d.tolist() == [[46893, 0]]
+
+
+def test_delay_table_properties():
+    b = synth.band_1000_1500(512)
+    f = b.freq
+    dms = np.arange(0, 200, 7.5)
+    d = O.delay_table(f, 0, 512, f[-1], float(b.tsamp), 1, dms)
+    assert np.all(d[:, -1] == 0)                  # zero at the reference channel
+    assert np.all(d[0] == 0)                      # DM 0
+    assert np.all(np.diff(d, axis=0) >= 0)        # monotone in DM
+    assert np.all(np.diff(d, axis=1) <= 0)        # monotone in frequency
+    d2 = O.delay_table(f, 0, 512, f[-1], float(b.tsamp), 3, dms)
+    assert np.array_equal(d2, d // 3)             # truncating division (d >= 0)
+
+
+def test_delay_c2_max_matches_config():
+    b = synth.band_1000_1500(4096)
+    d = O.delay_table(b.freq, 0, 4096, b.f_top, float(b.tsamp), 1, [3000.0])
+    assert d.max() == 140618 == synth.max_delay(b, 3000.0)
+
+
+def test_residuals_zero_at_group_start_and_match_delay():
+    b = synth.band_1000_1500(256)
+    f = b.freq
+    sbw, ndm0 = 32, 8
+    res = O.residual_table(f, 0, 256, sbw, f[-1], float(b.tsamp), 1, 0.0, 0.5, 40, ndm0)
+    assert np.all(res[::ndm0] == 0)
+    # with ds = 1, floor(x + 0.5) equals round-half-away for x >= 0, so the
+    # residual equals the delay of the subband centre at dm_m - dm_j
+    for m in (1, 5, 13, 39):
+        ddm = (m % ndm0) * 0.5
+        for sb in range(256 // sbw):
+            mid = (sb * sbw + sb * sbw + sbw - 1) // 2
+            d = O.delay_table(f, mid, mid + 1, f[-1], float(b.tsamp), 1, [ddm])[0, 0]
+            assert res[m, sb] == d
+
+
+# ------------------------------------------------------------------ binning --
+
+@pytest.mark.parametrize("ds", [2, 3, 4])
+def test_binning_u8_exhaustive_against_integer_sqrt(ds):
+    # out = floor(255*sqrt(S/(255*cnt))) = isqrt(floor(255*S/cnt)) exactly
+    sums = np.arange(0, 255 * ds + 1)
+    # build rows whose bin sums are exactly `sums`
+    x = np.zeros((len(sums) * ds, 1), np.uint8)
+    for i, s in enumerate(sums):
+        rem = int(s)
+        for k in range(ds):
+            v = min(255, rem)
+            x[i * ds + k, 0] = v
+            rem -= v
+    out = O.bin_time(x, ds)[:, 0]
+    exp = np.array([math.isqrt((255 * int(s)) // ds) for s in sums])
+    assert np.array_equal(out.astype(np.int64), exp)
+
+
+def test_binning_partial_last_bin():
+    x = np.array([[200], [100], [50]], np.uint8)
+    out = O.bin_time(x, 2)[:, 0]
+    assert out.tolist() == [math.isqrt(255 * 300 // 2), math.isqrt(255 * 50 // 1)]
+
+
+# --------------------------------------------------------------------- IQRM --
+
+def test_iqrm_lags_closed_form():
+    assert O.iqrm_lags(5, 1.5) == [1, -1, 2, -2, 3, -3, 4, -4]
+    assert O.iqrm_lags(30, 1.5) == [v for l in (1, 2, 3, 4, 6, 9, 13, 19, 28) for v in (l, -l)]
+    assert O.iqrm_radius(5.5 / 4095, 4095) == 5
+    assert O.iqrm_radius(0.0, 100) == 1
+
+
+def test_iqrm_windows_closed_form():
+    t0, t1 = O.iqrm_windows(100, 0, 0, True)
+    assert (t0.tolist(), t1.tolist()) == ([0], [100])
+    t0, t1 = O.iqrm_windows(100, 30, 20, True)
+    assert list(zip(t0.tolist(), t1.tolist())) == [(0, 30), (20, 50), (40, 70), (60, 90), (80, 100)]
+    t0, t1 = O.iqrm_windows(100, 30, 0, False)
+    assert list(zip(t0.tolist(), t1.tolist())) == [(0, 30), (30, 60), (60, 90)]
+    t0, t1 = O.iqrm_windows(93, 30, 30, True)   # tail of 3 samples (<= 4) is dropped
+    assert t1.tolist() == [30, 60, 90]
+
+
+@pytest.mark.parametrize("flavour", [O.IQRM_GPU, O.IQRM_CPU])
+def test_iqrm_special_cases(flavour):
+    x = np.full(200, 7.0, np.float32)
+    assert O.iqrm_on_stat(x, 0.1, 3.0, 1.5, flavour).sum() == 0   # IQR 0: nothing flagged
+    ramp = np.linspace(10, 12, 301).astype(np.float32)
+    ramp[137] += 50.0
+    m = O.iqrm_on_stat(ramp, 0.05, 3.0, 1.5, flavour)
+    assert np.flatnonzero(m).tolist() == [137]
+
+
+@needs_ref
+def test_iqrm_cpu_flavour_equals_reference_iqrm_hpp():
+    rng = np.random.default_rng(11)
+    for _ in range(200):
+        C_ = int(rng.integers(2, 700))
+        x = rng.normal(10, 1, C_).astype(np.float32)
+        k = rng.integers(0, C_, size=int(rng.integers(0, 12)))
+        x[k] += rng.uniform(2, 30, size=len(k)).astype(np.float32)
+        rf, ns = float(rng.uniform(0.003, 0.3)), float(rng.uniform(1.0, 4.0))
+        mine = O.iqrm_on_stat(x, rf, ns, 1.5, O.IQRM_CPU)
+        ref = np.zeros(C_, np.uint8)
+        REF.afref_iqrm_on_stat(O._p(x), C_, rf, ns, 1.5, O._p(ref))
+        assert np.array_equal(mine, ref)
+
+
+@needs_ref
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_iqrm_windows_and_stats_equal_reference(dtype):
+    rng = np.random.default_rng(5)
+    vmax = np.iinfo(dtype).max
+    for trial in range(25):
+        n, nc = int(rng.integers(40, 1500)), int(rng.integers(8, 260))
+        x = np.clip(np.round(rng.normal(vmax / 2, vmax / 16, (n, nc))), 0, vmax).astype(dtype)
+        hot = rng.choice(nc, size=max(1, nc // 20), replace=False)
+        x[:, hot] = np.clip(x[:, hot].astype(np.int64) * 2, 0, vmax).astype(dtype)
+        c0 = int(rng.integers(0, nc // 3))
+        c1 = int(rng.integers(c0 + 2, nc + 1))
+        tsamp = 1e-3
+        win = float(rng.choice([0.0, 0.05, 0.3]))
+        hop = float(rng.choice([0.0, 0.02, 0.1]))
+        W = int(math.floor(win / tsamp)) if win > 0 else 0
+        H = int(math.floor(hop / tsamp)) if hop > 0 else W
+        mode = int(rng.integers(0, 2))
+        m, t0, t1 = O.iqrm(x, c0, c1, mode, 0.1, 3.0, 1.5, W, H, True, O.IQRM_CPU)
+        cap = 4096
+        mr = np.zeros((cap, nc), np.uint8)
+        a = np.zeros(cap, np.uint32)
+        b = np.zeros(cap, np.uint32)
+        fn = REF.afref_iqrm_u8 if dtype == np.uint8 else REF.afref_iqrm_u16
+        nw = fn(O._p(x), n, nc, c0, c1, tsamp, mode, 0.1, 3.0, 1.5, win, hop, 1,
+                O._p(mr), O._p(a), O._p(b), cap)
+        assert nw == len(t0)
+        assert np.array_equal(a[:nw], t0) and np.array_equal(b[:nw], t1)
+        assert np.array_equal(mr[:nw], m)
+
+
+def test_iqrm_gpu_flavour_equals_cpu_where_precisions_agree():
+    # mode 0 (same mean formula) and C = 4k+1 (integer quartile positions):
+    # the float and double percentile paths coincide, so must the masks.
+    rng = np.random.default_rng(3)
+    for _ in range(60):
+        C_ = 4 * int(rng.integers(2, 150)) + 1
+        n = int(rng.integers(20, 200))
+        x = np.clip(np.round(rng.normal(128, 16, (n, C_))), 0, 255).astype(np.uint8)
+        hot = rng.choice(C_, size=max(1, C_ // 25), replace=False)
+        x[:, hot] = np.clip(x[:, hot].astype(np.int64) + 60, 0, 255).astype(np.uint8)
+        a, _, _ = O.iqrm(x, 0, C_, 0, 0.1, 3.0, 1.5, 0, 0, True, O.IQRM_GPU)
+        b, _, _ = O.iqrm(x, 0, C_, 0, 0.1, 3.0, 1.5, 0, 0, True, O.IQRM_CPU)
+        assert np.array_equal(a, b)
+
+
+def test_iqrm_std_flavours_agree_on_exact_stats():
+    # std of 0/v patterns is exact in both one-pass and two-pass forms
+    # (a 0/v alternation has std v/2 exactly)
+    C_, n = 41, 64
+    x = np.zeros((n, C_), np.uint8)
+    x[::2, :] = (8 + 2 * (np.arange(C_) % 7)).astype(np.uint8)
+    x[::2, 17] = 200
+    a, _, _ = O.iqrm(x, 0, C_, 1, 0.2, 3.0, 1.5, 0, 0, True, O.IQRM_GPU)
+    b, _, _ = O.iqrm(x, 0, C_, 1, 0.2, 3.0, 1.5, 0, 0, True, O.IQRM_CPU)
+    assert np.array_equal(a, b) and a[0, 17] == 1 and a.sum() == 1
+
+
+# --------------------------------------------------------- masks, zero-DM --
+
+@needs_ref
+def test_window_mask_application_equals_reference():
+    rng = np.random.default_rng(9)
+    x = rng.integers(1, 256, (300, 77)).astype(np.uint8)
+    masks = (rng.random((4, 77)) < 0.1).astype(np.uint8)
+    t0 = np.array([0, 50, 120, 290], np.uint32)
+    t1 = np.array([60, 130, 300, 300], np.uint32)
+    a = x.copy()
+    O.apply_window_masks(a, masks, t0, t1)
+    b = x.copy()
+    REF.afref_mask_u8(O._p(b), 77, 300, O._p(masks), O._p(t0), O._p(t1), 4)
+    assert np.array_equal(a, b)
+
+
+@needs_ref
+def test_static_mask_equals_reference_parser_and_marker():
+    x = np.full((50, 40), 9, np.uint8)
+    with tempfile.NamedTemporaryFile("w", suffix=".txt", delete=False) as fh:
+        fh.write("3 5\n\n  39\t-2 40 7\n")
+        path = fh.name
+    try:
+        b = x.copy()
+        bad = np.zeros(16, np.int32)
+        n = REF.afref_mark_rfi_u8(O._p(b), 40, 50, path.encode(), O._p(bad), 16)
+        parsed = bad[:n].tolist()
+        assert parsed == [3, 5, 39, -2, 40, 7]
+        a = x.copy()
+        O.static_mask(a, parsed)
+        assert np.array_equal(a, b)
+    finally:
+        os.unlink(path)
+
+
+def test_zero_dm_closed_forms():
+    x = np.array([[10, 20, 0, 30, 99],      # mean over non-zero of cols 0..3 = 20
+                  [7, 7, 7, 7, 7],          # all equal -> all zero
+                  [0, 0, 0, 0, 5]], np.uint8)  # no non-zero in range -> unchanged
+    O.zero_dm(x, 0, 4)
+    assert x.tolist() == [[0, 0, 0, 10, 99], [0, 0, 0, 0, 7], [0, 0, 0, 0, 5]]
+    y = np.array([[1, 2]], np.uint8)        # mean 1.5 -> 2-1.5 = 0.5 truncates to 0
+    O.zero_dm(y, 0, 2)
+    assert y.tolist() == [[0, 0]]
+    z = np.array([[250, 4, 1]], np.uint8)   # mean 85 (float 255/3) -> 165.0 exactly
+    O.zero_dm(z, 0, 3)
+    assert z.tolist() == [[165, 0, 0]]
+
+
+# -------------------------------------------------------------- dedispersion --
+
+def _tiny_case(seed, nchan=16, nsamp=256, ndm=8):
+    rng = np.random.default_rng(seed)
+    b = small_band(nchan, 1200.0, 1500.0, 1e-3)
+    x = rng.integers(0, 256, (nsamp, nchan)).astype(np.uint8)
+    dms = np.arange(ndm) * 40.0
+    d = O.delay_table(b.freq, 0, nchan, b.f_top, float(b.tsamp), 1, dms)
+    return b, x, dms, d
+
+
+def test_direct_brute_force():
+    b, x, dms, d = _tiny_case(1)
+    out = O.dedisperse_direct(x, 0, 16, d, 256)
+    exp = np.zeros_like(out)
+    for i in range(len(dms)):
+        for t in range(256):
+            s = 0
+            for c in range(16):
+                if t + d[i, c] < 256:
+                    s += int(x[t + d[i, c], c])
+            exp[i, t] = s
+    assert np.array_equal(out, exp)
+
+
+def test_direct_closed_forms():
+    b, x, dms, d = _tiny_case(2)
+    k = np.full_like(x, 3)
+    out = O.dedisperse_direct(k, 0, 16, d, 256)
+    valid = (np.arange(256)[None, :, None] + d[:, None, :] < 256).sum(-1)
+    assert np.array_equal(out, 3 * valid)                       # constant input
+    assert np.array_equal(out[0], 3 * np.full(256, 16))         # DM 0 row
+    assert np.array_equal(O.dedisperse_direct(x, 0, 16, d, 256)[0],
+                          x.astype(np.uint32).sum(1))           # DM 0 = channel sums
+    imp = np.zeros_like(x)
+    imp[200, 5] = 1
+    o = O.dedisperse_direct(imp, 0, 16, d, 256)
+    for i in range(len(dms)):
+        assert np.flatnonzero(o[i]).tolist() == [200 - d[i, 5]]
+    a = O.dedisperse_direct(x, 0, 16, d, 256)
+    y = np.random.default_rng(3).integers(0, 100, x.shape).astype(np.uint8)
+    assert np.array_equal(O.dedisperse_direct((x // 2 + y // 2).astype(np.uint8), 0, 16, d, 256),
+                          O.dedisperse_direct((x // 2).astype(np.uint8), 0, 16, d, 256)
+                          + O.dedisperse_direct((y // 2).astype(np.uint8), 0, 16, d, 256))
+    assert a.dtype == np.uint32
+
+
+def test_subband_reduces_to_direct():
+    b, x, dms, d = _tiny_case(4, nchan=24, nsamp=300, ndm=12)
+    f = b.freq
+    step = 13.0
+    d_direct = O.delay_table(f, 0, 24, f[-1], float(b.tsamp), 1, np.arange(12) * step)
+    direct = O.dedisperse_direct(x, 0, 24, d_direct, 300)
+    for sbw in (1, 24):   # one channel per subband, or one subband of everything
+        res = O.residual_table(f, 0, 24, sbw, f[-1], float(b.tsamp), 1, 0.0, step, 12, 1)
+        assert np.all(res == 0)
+        sub = O.dedisperse_subband(x, 0, 24, sbw, d_direct, res, 1, 300)
+        assert np.array_equal(sub, direct)
+
+
+def test_subband_brute_force_and_rows():
+    rng = np.random.default_rng(8)
+    b = small_band(20, 1100.0, 1500.0, 5e-4)
+    f = b.freq
+    x = rng.integers(0, 256, (400, 20)).astype(np.uint8)
+    ndm, ndm0, sbw, step = 10, 4, 6, 9.0
+    nnom = (ndm + ndm0 - 1) // ndm0
+    d1 = O.delay_table(f, 0, 20, f[-1], float(b.tsamp), 1, np.arange(nnom) * ndm0 * step)
+    res = O.residual_table(f, 0, 20, sbw, f[-1], float(b.tsamp), 1, 0.0, step, ndm, ndm0)
+    out = O.dedisperse_subband(x, 0, 20, sbw, d1, res, ndm0, 400)
+    exp = np.zeros_like(out)
+    for m in range(ndm):
+        j = m // ndm0
+        for t in range(400):
+            s = 0
+            for sb in range((20 + sbw - 1) // sbw):
+                ts = t + res[m, sb]
+                if ts >= 400:
+                    continue
+                for c in range(sb * sbw, min(sb * sbw + sbw, 20)):
+                    if ts + d1[j, c] < 400:
+                        s += int(x[ts + d1[j, c], c])
+            exp[m, t] = s
+    assert np.array_equal(out, exp)
+    rows = [7, 0, 3]
+    assert np.array_equal(O.dedisperse_subband_rows(x, 0, 20, sbw, d1, res, ndm0, 400, rows),
+                          out[rows])
+
+
+def test_cpucal_restatement_equals_direct():
+    b, x, dms, d = _tiny_case(6, nchan=32, nsamp=500, ndm=9)
+    cs, ce = 3, 30
+    dd = O.delay_table(b.freq, cs, ce, b.f_top, float(b.tsamp), 1, dms)
+    a = O.cpucal_direct_u8(x, cs, ce, dd, 0, 500, 1, nthreads=2)
+    bb = O.dedisperse_direct(x, cs, ce, dd, 500)
+    assert np.array_equal(a, bb.astype(np.uint64))
+    # decimation for ds > 1: reads native samples ti*ds + d
+    dec = O.cpucal_direct_u8(x, cs, ce, dd, 0, 500, 4, nthreads=2)
+    assert np.array_equal(dec, bb[:, ::4].astype(np.uint64))
