@@ -104,3 +104,48 @@ def band_1000_1500(nchan: int) -> Band:
 def max_delay(band: Band, dm_max: float) -> int:
     f = band.freq
     return int(delay_samples(f[:1], band.f_top, dm_max, float(band.tsamp))[0])
+
+
+def noise_u8_threaded(seed: int, nsamp: int, nchan: int, mean=128.0, sigma=16.0,
+                      threads: int = 8, rows_per_block: int = 8192) -> np.ndarray:
+    """clip(round(N(mean, sigma)), 0, 255) as uint8, generated in fixed blocks of
+    rows from spawned seeds (the result does not depend on the thread count)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    out = np.empty((nsamp, nchan), np.uint8)
+    nblk = (nsamp + rows_per_block - 1) // rows_per_block
+    seeds = np.random.SeedSequence(seed).spawn(nblk)
+
+    def work(b):
+        r0 = b * rows_per_block
+        r1 = min(nsamp, r0 + rows_per_block)
+        g = np.random.default_rng(seeds[b])
+        v = g.standard_normal((r1 - r0, nchan), dtype=np.float32)
+        v *= np.float32(sigma)
+        v += np.float32(mean)
+        np.rint(v, out=v)
+        np.clip(v, 0, 255, out=v)
+        out[r0:r1] = v.astype(np.uint8)
+
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        list(ex.map(work, range(nblk)))
+    return out
+
+
+def chunk_4096(seed: int = 2, threads: int = 8):
+    """The 4096-channel u8 chunk of the bench: 0.5 s of output (10173 samples)
+    plus the DM 0-3000 overlap tail, with narrowband RFI (5% of channels,
+    amplitude 40, periods 8-4096 samples), 10 channels stuck at 255 and pulses
+    at DM 150/1200/2800.  Returns (x [nsamp][4096] u8, band, nout, dmax)."""
+    band = band_1000_1500(4096)
+    tsamp = float(band.tsamp)
+    nout = int(math.ceil(0.5 / tsamp))
+    dmax = max_delay(band, 3000.0)
+    nsamp = nout + dmax
+    x = noise_u8_threaded(seed, nsamp, 4096, threads=threads)
+    rng = np.random.default_rng(seed + 1)
+    inject_narrowband_rfi(x, rng, frac=0.05, amp=40.0, n_stuck=10, pmin=8.0, pmax=4096.0)
+    f = band.freq
+    for dm, t0 in ((150.0, 2000), (1200.0, 5000), (2800.0, 8000)):
+        inject_pulse(x, f, band.f_top, tsamp, dm, t0, 1e-3, 48.0)
+    return x, band, nout, dmax

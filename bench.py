@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X dedispersion path on the 4096-channel u8 chunk.
+
+Workload (one step = one plain call on one chunk, input resident in HBM):
+  8-bit filterbank, 4096 channels over 1000-1500 MHz (top first, flipped to
+  ascending), tsamp 49.152 us; a 0.5 s chunk of 10173 output samples plus the
+  140618-sample DM-overlap tail (150791 input rows); N(128,16) noise, 5% of the
+  channels with sinusoidal RFI (amplitude 40, periods 8-4096 samples), 10
+  channels stuck at 255, pulses at DM 150/1200/2800; IQRM (std, radius 5
+  channels, 3 sigma, one window), zero-DM; subband dedispersion with 32
+  subbands of 128 channels, 32 fine DMs per coarse DM, DM 0-3000 step 0.5
+  (6001 trials).
+
+Prints one JSON line (see the repository's bench contract).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "DM-trial×samples/s at 4096ch uint8; p99 chunk latency, batched/served vs plain"
+UNIT = "DM-trial*samples/s"
+HBM_PEAK_GBS = 8000.0
+VALU_ADD_PEAK = 256 * 64 * 2.4e9     # 32-bit adds/s (one per lane per clock)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="target CPU work for the baseline sample")
+    return ap.parse_args()
+
+
+def c2_spec(band, nsamp, nout):
+    from astroflow_amd import capi
+    f = band.freq
+    nchan = band.nchan
+    radius_frac = float(np.float32(5.5 / 4095))   # floor(radius_frac * 4095) = 5 channels
+    return capi.DedispSpec(
+        dtype=np.uint8, nchan=nchan, nsamp=nsamp, freq=f, tsamp=float(band.tsamp),
+        dm_low=0.0, dm_step=0.5, ndm=6001, dd_c0=0, dd_c1=nchan, f_ref=float(f[-1]),
+        ds=1, nout=nout, method="subband", sbw=128, ndm0=32,
+        use_iqrm=True, iqrm_c0=0, iqrm_c1=nchan - 1, iqrm_mode=1,
+        iqrm_radius_frac=radius_frac, iqrm_nsigma=3.0, iqrm_geofactor=1.5,
+        iqrm_W=0, iqrm_H=0, iqrm_include_tail=True,
+        use_zero_dm=True, zdm_c0=0, zdm_c1=nchan - 1)
+
+
+def make_workload(threads=8):
+    """Seeded synthetic chunk on the host; returns (x, spec, info)."""
+    from astroflow_amd import synth
+    x, band, nout, dmax = synth.chunk_4096(seed=2, threads=threads)
+    spec = c2_spec(band, x.shape[0], nout)
+    info = dict(nchan=band.nchan, nout=nout, dmax=dmax, nsamp=x.shape[0], ndm=6001, ds=1)
+    return x, spec, info
+
+
+def cpu_baseline(x_host: np.ndarray, info, target_s: float):
+    """The reference CPU path (cpucal direct loop nest, OpenMP) timed on a DM
+    subset of the same chunk; returns the rate in DM-trial*samples/s."""
+    from astroflow_amd import synth
+    from oracle import oracle as O   # CPU baseline leg: the oracle's cpucal restatement
+
+    band = synth.band_1000_1500(info["nchan"])
+    f = band.freq
+    nchan, nout = info["nchan"], info["nout"]
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    threads = max(1, min(threads, 16))
+    all_dms = np.arange(info["ndm"]) * 0.5
+
+    def run(k):
+        idx = np.linspace(0, info["ndm"] - 1, k).round().astype(np.int64)
+        # cpucal excludes the top channel: channels [chan_start, chan_end)
+        d = O.delay_table(f, 0, nchan - 1, f[-1], float(band.tsamp), 1, all_dms[idx])
+        t0 = time.perf_counter()
+        O.cpucal_direct_u8(x_host, 0, nchan - 1, d, 0, nout, 1, nthreads=threads)
+        return time.perf_counter() - t0
+
+    k = 8
+    dt = run(k)
+    k2 = int(min(info["ndm"], max(8, round(k * target_s / max(dt, 1e-3)))))
+    reps = []
+    for _ in range(3):
+        reps.append(run(k2))
+        if sum(reps) > 2.5 * target_s:
+            break
+    med = float(np.median(reps))
+    rate = k2 * nout / med
+    return {"value": rate, "unit": UNIT, "cores": threads, "kind": "port",
+            "sample": f"{k2} of {info['ndm']} DM trials (evenly spaced over 0-3000) x {nout} "
+                      f"samples x {nchan - 1} channels, cpucal direct loop nest "
+                      f"(include/cpucal.hpp:167-186), u64 accumulation, median of {len(reps)} runs, "
+                      f"{med:.2f} s each"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is described in DESIGN.md, not run)"}))
+        sys.exit(2)
+    from astroflow_amd import capi
+
+    if capi.device_count() < 1:
+        print(json.dumps({"error": "no GPU visible"}))
+        sys.exit(1)
+    capi.check(capi.lib().af_set_device(0))
+    threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)))
+    x, spec, info = make_workload(threads)
+    plan = capi.Plan(spec)
+    d_in = capi.DeviceBuffer(x.nbytes)
+    d_in.upload(x)
+    d_out = capi.DeviceBuffer(info["ndm"] * info["nout"] * 4)
+
+    for _ in range(a.warmup):
+        plan.execute(d_in.ptr, d_out.ptr)
+    capi.sync()
+
+    plan.set_profiling(True)
+    capi.sync()
+    t_start = time.perf_counter()
+    for _ in range(a.steps):
+        plan.execute(d_in.ptr, d_out.ptr)
+    capi.sync()
+    t_end = time.perf_counter()
+    stages = plan.stage_times()
+    plan.set_profiling(False)
+
+    ms = (t_end - t_start) * 1e3 / a.steps
+    work = info["ndm"] * info["nout"] * info["ds"]
+    value = work / (ms / 1e3)
+
+    # per-call latency of the plain path (synchronous calls)
+    call_ms = []
+    for _ in range(max(5, min(a.steps, 20))):
+        t0 = time.perf_counter()
+        plan.execute(d_in.ptr, d_out.ptr)
+        capi.sync()
+        call_ms.append((time.perf_counter() - t0) * 1e3)
+
+    b_in = info["nchan"] * (info["nout"] * info["ds"] + info["dmax"]) * 1
+    b_out = info["ndm"] * info["nout"] * 4
+    b_alg = b_in + b_out
+    dom_ms = stages["dedisp"]
+    achieved = b_alg / (dom_ms / 1e3) / 1e9
+    adds = plan.info.adds
+
+    res = {
+        "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8", "data": "synthetic",
+        "config": {"workload": "single 0.5 s chunk, subband dedispersion + IQRM + zero-DM: "
+                               "4096 ch u8 1000-1500 MHz, 10173 out + 140618 tail samples, "
+                               "DM 0-3000 step 0.5 (6001 trials), 32 subbands x 128 ch, "
+                               "32 fine DMs per coarse DM",
+                   "nchan": info["nchan"], "nout": info["nout"], "nsamp_in": info["nsamp"],
+                   "ndm": info["ndm"], "nsub": 32, "ndm0": 32, "path": "plain single call"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_dd_u8 (subband stage 1)",
+                     "kernel_ms": dom_ms, "alg_bytes": b_alg},
+        "path_roofline": {"alg_bytes": b_alg, "GB_s": b_alg / (ms / 1e3) / 1e9,
+                          "hbm_frac": b_alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                          "adds": adds, "valu_add_frac": adds / (ms / 1e3) / VALU_ADD_PEAK},
+        "stage_ms": {k: v for k, v in stages.items() if k != "n"},
+        "latency_ms": {"p50": float(np.percentile(call_ms, 50)),
+                       "p99": float(np.percentile(call_ms, 99)),
+                       "chunk_duration_ms": info["nout"] * 49.152e-3},
+    }
+    if not a.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(x, info, a.cpu_seconds)
+        res["speedup_vs_cpu"] = value / res["cpu_baseline"]["value"]
+    else:
+        res["cpu_baseline"] = None
+    plan.close()
+    d_in.free()
+    d_out.free()
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -215,65 +215,78 @@ int afo_iqrm_radius(float radius_frac, int C) {
 void afo_iqrm_stat(const void *data, int dtype, int nchan, uint32_t t0,
                    uint32_t t1, int c0, int c1, int mode, int flavour,
                    float *x) {
-  uint32_t W = t1 - t0;
-  for (int c = c0; c < c1; ++c) {
-    float *xo = &x[c - c0];
-    if (dtype == AFO_F32) {
-      const float *f = (const float *)data;
-      double sum = 0.0, sumsq = 0.0;
-      for (uint32_t t = t0; t < t1; ++t) {
+  /* Row-major traversal with per-channel accumulators: each channel still
+   * sums its samples in ascending time order. */
+  const uint32_t W = t1 - t0;
+  const int C = c1 - c0;
+  double *ds = (double *)calloc((size_t)C, sizeof(double));
+  double *dq = (double *)calloc((size_t)C, sizeof(double));
+  uint64_t *us = (uint64_t *)calloc((size_t)C, sizeof(uint64_t));
+  u128 *uq = (u128 *)calloc((size_t)C, sizeof(u128));
+  if (dtype == AFO_F32) {
+    const float *f = (const float *)data;
+    for (uint32_t t = t0; t < t1; ++t)
+      for (int c = c0; c < c1; ++c) {
         double v = (double)f[(size_t)t * nchan + c];
-        sum += v;
-        sumsq += v * v;
+        ds[c - c0] += v;
+        dq[c - c0] += v * v;
       }
-      double mean = sum / (double)W;
-      if (mode == 0) { *xo = (float)mean; continue; }
-      double var;
-      if (flavour == AFO_IQRM_CPU) {
-        double s2 = 0.0;
-        for (uint32_t t = t0; t < t1; ++t) {
-          double v = (double)f[(size_t)t * nchan + c] - mean;
-          s2 += v * v;
-        }
-        var = s2 / (double)W;
-      } else {
-        var = sumsq / (double)W - mean * mean;
-      }
+    for (int k = 0; k < C; ++k) {
+      double mean = ds[k] / (double)W;
+      if (mode == 0) { x[k] = (float)mean; continue; }
+      double var = dq[k] / (double)W - mean * mean;
       if (var < 0.0) var = 0.0;
-      *xo = (float)sqrt(var);
-      continue;
+      x[k] = (float)sqrt(var);
     }
-    if (flavour == AFO_IQRM_CPU) {
-      /* include/iqrm.hpp:189-217: double accumulation, two-pass std */
-      double sum = 0.0;
-      for (uint32_t t = t0; t < t1; ++t) sum += (double)ld_u(data, dtype, (size_t)t * nchan + c);
-      double mean = sum / (double)W;
-      if (mode == 0) { *xo = (float)mean; continue; }
-      double s2 = 0.0;
-      for (uint32_t t = t0; t < t1; ++t) {
-        double v = (double)ld_u(data, dtype, (size_t)t * nchan + c) - mean;
-        s2 += v * v;
+    if (mode == 1 && flavour == AFO_IQRM_CPU) {
+      memset(dq, 0, sizeof(double) * C);
+      for (uint32_t t = t0; t < t1; ++t)
+        for (int c = c0; c < c1; ++c) {
+          double v = (double)f[(size_t)t * nchan + c] - ds[c - c0] / (double)W;
+          dq[c - c0] += v * v;
+        }
+      for (int k = 0; k < C; ++k) {
+        double var = dq[k] / (double)W;
+        x[k] = (float)sqrt(var > 0.0 ? var : 0.0);
       }
-      s2 /= (double)W;
-      *xo = (float)sqrt(s2 > 0.0 ? s2 : 0.0);
+    }
+  } else if (flavour == AFO_IQRM_CPU) {
+    /* include/iqrm.hpp:189-217: double accumulation, two-pass std */
+    for (uint32_t t = t0; t < t1; ++t)
+      for (int c = c0; c < c1; ++c) ds[c - c0] += (double)ld_u(data, dtype, (size_t)t * nchan + c);
+    for (int k = 0; k < C; ++k) ds[k] = ds[k] / (double)W;   /* mean */
+    if (mode == 0) {
+      for (int k = 0; k < C; ++k) x[k] = (float)ds[k];
     } else {
-      /* src/core/iqrm.cu:21-91: sum and sum of squares (exact integers
-       * here, which is what the reference's double sums equal for 8/16-bit
-       * data), then mean = S/W and var = S2/W - mean^2 in double. */
-      uint64_t sum = 0;
-      u128 sumsq = 0;
-      for (uint32_t t = t0; t < t1; ++t) {
-        uint64_t v = ld_u(data, dtype, (size_t)t * nchan + c);
-        sum += v;
-        sumsq += (u128)v * v;
+      for (uint32_t t = t0; t < t1; ++t)
+        for (int c = c0; c < c1; ++c) {
+          double v = (double)ld_u(data, dtype, (size_t)t * nchan + c) - ds[c - c0];
+          dq[c - c0] += v * v;
+        }
+      for (int k = 0; k < C; ++k) {
+        double s2 = dq[k] / (double)W;
+        x[k] = (float)sqrt(s2 > 0.0 ? s2 : 0.0);
       }
-      double mean = (double)sum / (double)W;
-      if (mode == 0) { *xo = (float)mean; continue; }
-      double var = (double)sumsq / (double)W - mean * mean;
+    }
+  } else {
+    /* src/core/iqrm.cu:21-91: sum and sum of squares (exact integers here,
+     * which is what the reference's double sums equal for 8/16-bit data),
+     * then mean = S/W and var = S2/W - mean^2 in double. */
+    for (uint32_t t = t0; t < t1; ++t)
+      for (int c = c0; c < c1; ++c) {
+        uint64_t v = ld_u(data, dtype, (size_t)t * nchan + c);
+        us[c - c0] += v;
+        uq[c - c0] += (u128)v * v;
+      }
+    for (int k = 0; k < C; ++k) {
+      double mean = (double)us[k] / (double)W;
+      if (mode == 0) { x[k] = (float)mean; continue; }
+      double var = (double)uq[k] / (double)W - mean * mean;
       if (var < 0.0) var = 0.0;
-      *xo = (float)sqrt(var);
+      x[k] = (float)sqrt(var);
     }
   }
+  free(ds); free(dq); free(us); free(uq);
 }
 
 static int cmp_float(const void *a, const void *b) {
