@@ -55,6 +55,12 @@ class PlanInfo(C.Structure):
                 ("adds", C.c_double)]
 
 
+class DeviceProps(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("arch", C.c_char * 32), ("cu_count", C.c_int),
+                ("clock_khz", C.c_int), ("lds_per_block", C.c_int), ("lds_per_cu", C.c_int),
+                ("l2_bytes", C.c_int), ("hbm_bytes", C.c_size_t)]
+
+
 class StageTimes(C.Structure):
     _fields_ = [("bin", C.c_float), ("stats", C.c_float), ("iqrm", C.c_float),
                 ("clean", C.c_float), ("dedisp", C.c_float), ("stage2", C.c_float),
@@ -75,6 +81,7 @@ SIGNATURES = {
     "af_plan_stage_times": (_I, [_P, C.POINTER(StageTimes)]),
     "af_execute": (_I, [_P, _P, _P, _P, _P]),
     "af_execute_host": (_I, [_P, _P, _P, _P]),
+    "af_device_query": (_I, [_I, C.POINTER(DeviceProps)]),
     "af_device_count": (_I, [C.POINTER(_I)]),
     "af_set_device": (_I, [_I]),
     "af_malloc": (_I, [C.POINTER(_P), _S]),
@@ -119,6 +126,13 @@ def device_count() -> int:
     n = C.c_int(0)
     check(lib().af_device_count(C.byref(n)), "af_device_count")
     return n.value
+
+
+def device_props(dev: int = 0) -> dict:
+    p = DeviceProps()
+    check(lib().af_device_query(dev, C.byref(p)), "af_device_query")
+    return {k: (getattr(p, k).decode() if isinstance(getattr(p, k), bytes) else getattr(p, k))
+            for k, _ in DeviceProps._fields_}
 
 
 def _ptr(a: np.ndarray):

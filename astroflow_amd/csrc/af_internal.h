@@ -132,7 +132,7 @@ struct af_plan {
   size_t nbin = 0, nout = 0, Lp = 0, L1 = 0;
   int nch = 0, ndm = 0, nsb = 0, ndm_nom = 0, maxdelay = 0, maxres = 0;
   int ntiles1 = 0;          // tiles of the first dedispersion stage
-  int s2_tile = 0, s2_lds = 0;
+  int s2_tile = 0, s2_lds = 0, maxlds = 0;
   double adds = 0;
   std::vector<int32_t> h_delays, h_res, h_maxres_j;
 
@@ -167,3 +167,16 @@ struct af_plan {
   int acc_n = 0;
   std::vector<void *> allocs;
 };
+
+namespace af {
+// 8-bit fast paths (af_kernels_u8.hip)
+constexpr int kStatRowsU8 = 256;   // rows per statistics item (16-bit lane sums)
+constexpr int kCleanRowsU8 = 32;   // rows per clean item (1024 threads x 8 rows x 16 ch)
+constexpr int kCleanU8MaxChan = 4096;
+hipError_t launch_seg_stats_u8(const uint8_t *x, int nchan, int c0, int c1, const RowItem *items,
+                               int nitems, uint2 *part, hipStream_t st);
+hipError_t launch_seg_reduce_u8(const uint2 *part, const int *seg_items, int nseg, int Csub,
+                                Part *segsum, hipStream_t st);
+hipError_t launch_clean_u8(const CleanArgs &a, hipStream_t st);
+hipError_t launch_stage2_u16v(const Stage2Args &a, hipStream_t st);
+}  // namespace af

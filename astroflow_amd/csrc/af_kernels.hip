@@ -612,6 +612,8 @@ hipError_t launch_bin(int dtype, const void *in, void *out, size_t nsamp, int nc
 
 hipError_t launch_seg_stats(int dtype, const void *x, int nchan, int c0, int c1,
                             const RowItem *items, int nitems, Part *part, hipStream_t st) {
+  if (dtype == AF_U8)
+    return launch_seg_stats_u8((const uint8_t *)x, nchan, c0, c1, items, nitems, (uint2 *)part, st);
   const int nchunk = (c1 - c0 + 255) / 256;
   AF_DISPATCH(dtype, T,
               hipLaunchKernelGGL(k_seg_stats<T>, dim3((unsigned)(nitems * nchunk)), dim3(256),
@@ -621,6 +623,8 @@ hipError_t launch_seg_stats(int dtype, const void *x, int nchan, int c0, int c1,
 
 hipError_t launch_seg_reduce(int dtype, const Part *part, const int *seg_items, int nseg,
                              int Csub, Part *segsum, hipStream_t st) {
+  if (dtype == AF_U8)
+    return launch_seg_reduce_u8((const uint2 *)part, seg_items, nseg, Csub, segsum, st);
   const size_t n = (size_t)nseg * Csub;
   AF_DISPATCH(dtype, T,
               hipLaunchKernelGGL(k_seg_reduce<T>, dim3((unsigned)((n + 255) / 256)), dim3(256),
@@ -656,6 +660,7 @@ hipError_t launch_segmask(const uint8_t *masks, const Seg *segs, int nseg, int n
 }
 
 hipError_t launch_clean(int dtype, const CleanArgs &a, hipStream_t st) {
+  if (dtype == AF_U8 && a.nchan <= kCleanU8MaxChan) return launch_clean_u8(a, st);
   AF_DISPATCH(dtype, T,
               hipLaunchKernelGGL(k_clean<T>, dim3((unsigned)a.nitems), dim3(256), 0, st, a));
   return hipGetLastError();
@@ -694,6 +699,8 @@ hipError_t launch_dedisperse(int dtype, const DdArgs &a, hipStream_t st) {
 }
 
 hipError_t launch_stage2(int dtype, const Stage2Args &a, hipStream_t st) {
+  if (dtype == AF_U8 && a.inter16 && (a.tile == 512 || a.tile == 1024))
+    return launch_stage2_u16v(a, st);
   if (dtype == AF_U8 && a.inter16) {
     const int ntile = (int)((a.nout + a.tile - 1) / a.tile);
     const size_t nblk = (size_t)ntile * a.ndm_nom;

@@ -1,0 +1,348 @@
+// af_kernels_u8.hip -- 8-bit fast paths of the RFI passes and subband stage 2.
+//
+// Each thread owns 16 consecutive channels (one 16-byte load per row), so a
+// wave reads 1 KB of a row per instruction.  Compiled with -ffp-contract=off.
+#include "af_internal.h"
+
+namespace af {
+
+__device__ __forceinline__ uint32_t xcd_remap_u8(uint32_t b, uint32_t n) {
+  const uint32_t X = 8;
+  uint32_t q = n / X, r = n % X;
+  uint32_t xcd = b % X, k = b / X;
+  uint32_t start = xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
+  return start + k;
+}
+
+__device__ __forceinline__ uint4 ld16(const uint8_t *p) {
+  uint4 v;
+  __builtin_memcpy(&v, p, 16);
+  return v;
+}
+
+__device__ __forceinline__ uint32_t u4get(const uint4 &v, int k) {
+  return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
+}
+
+// Loads 16 channels [cb, cb+16) of one row; channels at or beyond `lim`
+// read as zero (and are never touched in memory).
+__device__ __forceinline__ uint4 load_row16(const uint8_t *row, int cb, int lim) {
+  if (cb + 16 <= lim) return ld16(row + cb);
+  uint32_t w[4] = {0, 0, 0, 0};
+  for (int k = 0; k < 16; ++k)
+    if (cb + k < lim) w[k >> 2] |= (uint32_t)row[cb + k] << (8 * (k & 3));
+  return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// ------------------------------------------------------ IQRM statistics --
+// Per (row block <= 256 rows, channel): sum and sum of squares.  Sums are
+// accumulated as packed 16-bit lanes (exact: <= 256*255), squares with one
+// multiply-add per byte.
+__global__ __launch_bounds__(256) void k_seg_stats_u8(const uint8_t *__restrict__ x, int nchan,
+                                                      int c0, int c1,
+                                                      const RowItem *__restrict__ items,
+                                                      uint2 *__restrict__ part) {
+  const int Csub = c1 - c0;
+  const int ngrp = (Csub + 15) / 16;
+  const int nchunk = (ngrp + 255) / 256;
+  const int item = blockIdx.x / nchunk;
+  const int g = (blockIdx.x % nchunk) * 256 + threadIdx.x;
+  if (g >= ngrp) return;
+  const int cb = c0 + 16 * g;
+  const RowItem it = items[item];
+  uint32_t se[4] = {0, 0, 0, 0}, so[4] = {0, 0, 0, 0};
+  uint32_t q[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) q[k] = 0;
+  const uint8_t *row = x + (size_t)it.r0 * nchan;
+#pragma unroll 2
+  for (uint32_t r = it.r0; r < it.r1; ++r, row += nchan) {
+    const uint4 w = load_row16(row, cb, c1);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t v = u4get(w, k);
+      se[k] += v & 0x00FF00FFu;
+      so[k] += __builtin_amdgcn_perm(0u, v, 0x0c030c01u);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const uint32_t e = (v >> (8 * b)) & 0xFFu;
+        q[4 * k + b] += e * e;
+      }
+    }
+  }
+  uint2 *o = part + (size_t)item * Csub + (cb - c0);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t s4[4] = {se[k] & 0xFFFFu, so[k] & 0xFFFFu, se[k] >> 16, so[k] >> 16};
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if (cb + 4 * k + b < c1) o[4 * k + b] = make_uint2(s4[b], q[4 * k + b]);
+  }
+}
+
+__global__ void k_seg_reduce_u8(const uint2 *__restrict__ part, const int *__restrict__ seg_items,
+                                int nseg, int Csub, Part *__restrict__ segsum) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)nseg * Csub) return;
+  const int s = (int)(idx / Csub), c = (int)(idx % Csub);
+  uint64_t a = 0, b = 0;
+  for (int i = seg_items[s]; i < seg_items[s + 1]; ++i) {
+    const uint2 p = part[(size_t)i * Csub + c];
+    a += p.x;
+    b += p.y;
+  }
+  segsum[idx] = Part{a, b, 0};
+}
+
+// ------------------------------------------------ clean + transpose (u8) --
+// One 1024-thread block per 32-row item.  Lane l of wave w owns channel
+// group g = 16*w + (l >> 2) (16 channels) and row quarter q = l & 3 (rows
+// 8q..8q+7): 8 16-byte loads kept in registers.  Pass 1: IQRM window mask,
+// per-row non-zero sum/count over the zero-DM range (dot4 byte sums), wave
+// reduction, LDS atomics.  Pass 2: x = trunc(max((float)x - m, 0)) inside
+// the zero-DM range, static mask, 4x4 byte transposes with v_perm, and one
+// 8-byte store per channel (the four lanes of a group write 32 contiguous
+// bytes of that channel's row).
+__device__ __forceinline__ uint32_t zdm4(uint32_t w, float m) {
+  uint32_t o = 0;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    float f = (float)((w >> (8 * b)) & 0xFFu) - m;
+    f = f < 0.0f ? 0.0f : f;
+    o |= ((uint32_t)f) << (8 * b);
+  }
+  return o;
+}
+
+__device__ __forceinline__ uint32_t nonzero_bytes(uint32_t w) {
+  const uint32_t t = ((w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | w;   // bit 7 of each byte: byte != 0
+  return __builtin_popcount(t & 0x80808080u);
+}
+
+__global__ __launch_bounds__(1024) void k_clean_u8(CleanArgs a) {
+  __shared__ uint32_t s_sum[32], s_cnt[32];
+  __shared__ float s_m[32];
+  const uint32_t bid = xcd_remap_u8(blockIdx.x, gridDim.x);
+  const RowItem it = a.items[bid];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int q = lane & 3;
+  const int g = wave * 16 + (lane >> 2);
+  const int cb = 16 * g;
+  const int nrows = (int)(it.r1 - it.r0);
+  const uint8_t *__restrict__ x = (const uint8_t *)a.x;
+  if (threadIdx.x < 32) {
+    s_sum[threadIdx.x] = 0;
+    s_cnt[threadIdx.x] = 0;
+  }
+  // per-thread channel masks (0xFF = keep / in range)
+  uint32_t km[4], zr[4], st[4];
+  const uint8_t *smask = a.segmask ? a.segmask + (size_t)it.seg * a.nchan : nullptr;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    uint32_t m1 = 0, m2 = 0, m3 = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int c = cb + 4 * k + b;
+      const bool inside = c < a.nchan;
+      const bool keep = inside && !(smask && smask[c]);
+      const bool zin = a.zdm && c >= a.zc0 && c < a.zc1;
+      const bool sk = inside && !(a.stat_mask && a.stat_mask[c]);
+      m1 |= (keep ? 0xFFu : 0u) << (8 * b);
+      m2 |= (zin ? 0xFFu : 0u) << (8 * b);
+      m3 |= (sk ? 0xFFu : 0u) << (8 * b);
+    }
+    km[k] = m1;
+    zr[k] = m2;
+    st[k] = m3;
+  }
+  uint4 w[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int r = 8 * q + i;
+    w[i] = make_uint4(0, 0, 0, 0);
+    if (r < nrows && cb < a.nchan) {
+      uint4 v = load_row16(x + (size_t)(it.r0 + r) * a.nchan, cb, a.nchan);
+      w[i] = make_uint4(v.x & km[0], v.y & km[1], v.z & km[2], v.w & km[3]);
+    }
+  }
+  __syncthreads();
+  if (a.zdm) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      uint32_t s = 0, n = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t z = u4get(w[i], k) & zr[k];
+        s = __builtin_amdgcn_udot4(z, 0x01010101u, s, false);
+        n += nonzero_bytes(z);
+      }
+      // per wave: s <= 1024*255 < 2^18, n <= 1024 < 2^11
+      uint32_t pk = s | (n << 18);
+      pk += __shfl_xor(pk, 4);
+      pk += __shfl_xor(pk, 8);
+      pk += __shfl_xor(pk, 16);
+      pk += __shfl_xor(pk, 32);
+      if (lane < 4) {
+        atomicAdd(&s_sum[8 * lane + i], pk & 0x3FFFFu);
+        atomicAdd(&s_cnt[8 * lane + i], pk >> 18);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < 32) {
+      const uint32_t n = s_cnt[threadIdx.x];
+      s_m[threadIdx.x] = n ? (float)s_sum[threadIdx.x] / (float)n : 0.0f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const float m = s_m[8 * q + i];
+      uint32_t o[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t v = u4get(w[i], k);
+        o[k] = (zdm4(v, m) & zr[k]) | (v & ~zr[k]);
+      }
+      w[i] = make_uint4(o[0] & st[0], o[1] & st[1], o[2] & st[2], o[3] & st[3]);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      w[i] = make_uint4(w[i].x & st[0], w[i].y & st[1], w[i].z & st[2], w[i].w & st[3]);
+  }
+  if (cb >= a.nchan) return;
+  // transpose: for channel quad k and row half h, rows 4h..4h+3
+  uint8_t *__restrict__ xT = (uint8_t *)a.xT;
+  const int rbase = 8 * q;
+  const bool full = rbase + 8 <= nrows;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    uint32_t lo[4], hi[4];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t ra = u4get(w[4 * h + 0], k), rb = u4get(w[4 * h + 1], k);
+      const uint32_t rc = u4get(w[4 * h + 2], k), rd = u4get(w[4 * h + 3], k);
+      const uint32_t t0 = __builtin_amdgcn_perm(rb, ra, 0x05010400u);
+      const uint32_t t1 = __builtin_amdgcn_perm(rb, ra, 0x07030602u);
+      const uint32_t t2 = __builtin_amdgcn_perm(rd, rc, 0x05010400u);
+      const uint32_t t3 = __builtin_amdgcn_perm(rd, rc, 0x07030602u);
+      uint32_t *dst = h ? hi : lo;
+      dst[0] = __builtin_amdgcn_perm(t2, t0, 0x05040100u);
+      dst[1] = __builtin_amdgcn_perm(t2, t0, 0x07060302u);
+      dst[2] = __builtin_amdgcn_perm(t3, t1, 0x05040100u);
+      dst[3] = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int c = cb + 4 * k + b;
+      if (c < a.dd_c0 || c >= a.dd_c1) continue;
+      uint8_t *dst = xT + (size_t)(c - a.dd_c0) * a.Lp + it.r0 + rbase;
+      if (full) {
+        const uint2 v = make_uint2(lo[b], hi[b]);
+        __builtin_memcpy(dst, &v, 8);
+      } else {
+        for (int r = 0; r < 8; ++r)
+          if (rbase + r < nrows) dst[r] = (uint8_t)(((r < 4 ? lo[b] : hi[b]) >> (8 * (r & 3))) & 0xFFu);
+      }
+    }
+  }
+}
+
+// --------------------------------------------- subband stage 2 (u16 in) --
+// One workgroup (512 threads) per (coarse DM group, tile of 64*P outputs):
+// the group's partial sums for the tile plus its largest residual are staged
+// in LDS with 16-byte loads; each wave then sums the subbands of one fine DM
+// at a time, each lane P consecutive outputs read as P/8 16-byte LDS loads.
+template <int P>
+__global__ __launch_bounds__(512) void k_stage2_u16v(Stage2Args a) {
+  extern __shared__ uint4 s2v[];
+  uint16_t *s2 = (uint16_t *)s2v;
+  constexpr int TILE = 64 * P;
+  const int ntile = (int)((a.nout + TILE - 1) / TILE);
+  const uint32_t lid = xcd_remap_u8(blockIdx.x, gridDim.x);
+  const int ti = (int)(lid % (uint32_t)ntile);
+  const int j = (int)(lid / (uint32_t)ntile);
+  const size_t t0 = (size_t)ti * TILE;
+  const int ext8 = (TILE + a.maxres_j[j] + 7) / 8;   // 16-byte chunks per subband row
+  const int lds8 = a.lds_len / 8;
+  const uint16_t *__restrict__ src = (const uint16_t *)a.inter + (size_t)j * a.nsb * a.L1 + t0;
+  for (int idx = threadIdx.x; idx < a.nsb * ext8; idx += blockDim.x) {
+    const int sb = idx / ext8, k = idx % ext8;
+    s2v[sb * lds8 + k] = *(const uint4 *)(src + (size_t)sb * a.L1 + 8 * k);
+  }
+  __syncthreads();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int nw = blockDim.x >> 6;
+  for (int mi = wave; mi < a.ndm0; mi += nw) {
+    const int m = j * a.ndm0 + mi;
+    if (m >= a.ndm) break;
+    const int32_t *__restrict__ rrow = a.res + (size_t)m * a.nsb;
+    uint32_t acc[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) acc[i] = 0;
+    for (int sb = 0; sb < a.nsb; ++sb) {
+      const uint16_t *p = s2 + sb * a.lds_len + lane * P + rrow[sb];
+#pragma unroll
+      for (int h = 0; h < P / 8; ++h) {
+        uint4 v;
+        __builtin_memcpy(&v, p + 8 * h, 16);
+        acc[8 * h + 0] += v.x & 0xFFFFu; acc[8 * h + 1] += v.x >> 16;
+        acc[8 * h + 2] += v.y & 0xFFFFu; acc[8 * h + 3] += v.y >> 16;
+        acc[8 * h + 4] += v.z & 0xFFFFu; acc[8 * h + 5] += v.z >> 16;
+        acc[8 * h + 6] += v.w & 0xFFFFu; acc[8 * h + 7] += v.w >> 16;
+      }
+    }
+    uint32_t *o = (uint32_t *)a.out + (size_t)m * a.nout + t0 + (size_t)lane * P;
+    if (t0 + (size_t)lane * P + P <= a.nout) {
+      __builtin_memcpy(o, acc, 4 * P);
+    } else {
+      for (int i = 0; i < P; ++i)
+        if (t0 + (size_t)lane * P + i < a.nout) o[i] = acc[i];
+    }
+  }
+}
+
+// ---------------------------------------------------------- launchers --
+
+hipError_t launch_seg_stats_u8(const uint8_t *x, int nchan, int c0, int c1, const RowItem *items,
+                               int nitems, uint2 *part, hipStream_t st) {
+  const int ngrp = (c1 - c0 + 15) / 16;
+  const int nchunk = (ngrp + 255) / 256;
+  hipLaunchKernelGGL(k_seg_stats_u8, dim3((unsigned)(nitems * nchunk)), dim3(256), 0, st, x,
+                     nchan, c0, c1, items, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_seg_reduce_u8(const uint2 *part, const int *seg_items, int nseg, int Csub,
+                                Part *segsum, hipStream_t st) {
+  const size_t n = (size_t)nseg * Csub;
+  hipLaunchKernelGGL(k_seg_reduce_u8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, part,
+                     seg_items, nseg, Csub, segsum);
+  return hipGetLastError();
+}
+
+hipError_t launch_clean_u8(const CleanArgs &a, hipStream_t st) {
+  hipLaunchKernelGGL(k_clean_u8, dim3((unsigned)a.nitems), dim3(1024), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_stage2_u16v(const Stage2Args &a, hipStream_t st) {
+  const int P = a.tile / 64;
+  const int ntile = (int)((a.nout + a.tile - 1) / a.tile);
+  const size_t nblk = (size_t)ntile * a.ndm_nom;
+  const size_t lds = (size_t)a.nsb * a.lds_len * sizeof(uint16_t);
+  if (P == 16) {
+    (void)hipFuncSetAttribute((const void *)k_stage2_u16v<16>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_stage2_u16v<16>, dim3((unsigned)nblk), dim3(512), lds, st, a);
+  } else if (P == 8) {
+    (void)hipFuncSetAttribute((const void *)k_stage2_u16v<8>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_stage2_u16v<8>, dim3((unsigned)nblk), dim3(512), lds, st, a);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace af

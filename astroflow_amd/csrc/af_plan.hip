@@ -228,12 +228,17 @@ static af_status build(af_plan *p) {
   p->maxdelay = (int)maxd;
 
   // ---- sizes of the dedispersion stages
+  // LDS per workgroup: the largest of the device's per-block limits
   int maxlds = 65536;
   {
-    int v = 0;
-    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device) == hipSuccess && v > 0)
-      maxlds = v;
+    const hipDeviceAttribute_t attrs[] = {hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                          hipDeviceAttributeSharedMemPerBlockOptin};
+    for (auto at : attrs) {
+      int v = 0;
+      if (hipDeviceGetAttribute(&v, at, p->device) == hipSuccess && v > maxlds) maxlds = v;
+    }
   }
+  p->maxlds = maxlds;
   size_t n_ext;
   if (d.method == AF_METHOD_DIRECT) {
     n_ext = p->nout;
@@ -242,17 +247,24 @@ static af_status build(af_plan *p) {
     const bool inter16 = d.dtype == AF_U8 && d.sb_nchan <= 256;
     int tile = 0;
     if (inter16) {
-      const int cand[] = {1024, 512, 256, 128, 64};
-      for (int t : cand) {
-        size_t need = (size_t)p->nsb * (size_t)(t + p->maxres) * 2;
-        if (need <= (size_t)std::min(maxlds, 65536)) { tile = t; break; }
+      auto lds_of = [&](int t) { return (size_t)p->nsb * (size_t)((t + p->maxres + 7) / 8 * 8) * 2; };
+      // 16-byte-load kernel: prefer two workgroups per CU, then one
+      const int big[] = {1024, 512};
+      for (int half = 2; half >= 1 && !tile; --half)
+        for (int t : big)
+          if (lds_of(t) <= (size_t)maxlds / half) { tile = t; break; }
+      // element-wise kernel within 64 KB
+      if (!tile) {
+        const int small[] = {256, 128, 64};
+        for (int t : small)
+          if (lds_of(t) <= (size_t)std::min(maxlds, 65536)) { tile = t; break; }
       }
     }
     if (tile == 0) tile = 512;  // generic stage 2 (no LDS)
     p->s2_tile = tile;
-    p->s2_lds = tile + p->maxres;
+    p->s2_lds = (tile + p->maxres + 7) / 8 * 8;
     size_t ntile2 = (p->nout + tile - 1) / tile;
-    n_ext = ntile2 * (size_t)tile + (size_t)p->maxres;
+    n_ext = ntile2 * (size_t)tile + (size_t)p->maxres + 8;
     p->adds = (double)p->ndm_nom * p->nch * (double)(p->nout + p->maxres) +
               (double)p->ndm * p->nsb * (double)p->nout;
   }
@@ -316,13 +328,16 @@ static af_status build(af_plan *p) {
     p->wseg[2 * w] = (uint32_t)a;
     p->wseg[2 * w + 1] = (uint32_t)b + 1;
   }
+  const uint32_t stat_rows = d.dtype == AF_U8 ? kStatRowsU8 : kStatRows;
+  const uint32_t clean_rows =
+      (d.dtype == AF_U8 && d.nchan <= kCleanU8MaxChan) ? kCleanRowsU8 : kCleanRows;
   p->seg_items.assign(1, 0);
   for (int s = 0; s < p->nseg; ++s) {
-    for (uint32_t r = p->segs[s].t0; r < p->segs[s].t1; r += kStatRows)
-      p->stat_items.push_back(RowItem{(uint32_t)s, r, std::min<uint32_t>(r + kStatRows, p->segs[s].t1), 0});
+    for (uint32_t r = p->segs[s].t0; r < p->segs[s].t1; r += stat_rows)
+      p->stat_items.push_back(RowItem{(uint32_t)s, r, std::min<uint32_t>(r + stat_rows, p->segs[s].t1), 0});
     p->seg_items.push_back((int)p->stat_items.size());
-    for (uint32_t r = p->segs[s].t0; r < p->segs[s].t1; r += kCleanRows)
-      p->clean_items.push_back(RowItem{(uint32_t)s, r, std::min<uint32_t>(r + kCleanRows, p->segs[s].t1), 0});
+    for (uint32_t r = p->segs[s].t0; r < p->segs[s].t1; r += clean_rows)
+      p->clean_items.push_back(RowItem{(uint32_t)s, r, std::min<uint32_t>(r + clean_rows, p->segs[s].t1), 0});
   }
 
   // ---- device memory
@@ -651,6 +666,25 @@ af_status af_time_execute(af_plan *p, const void *d_in, void *d_out, int reps, f
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   return s;
+}
+
+af_status af_device_query(int dev, af_device_props *props) {
+  if (!props) return fail(AF_ERR_INVALID, "null argument");
+  std::memset(props, 0, sizeof(*props));
+  hipDeviceProp_t pr;
+  AF_HIP(hipGetDeviceProperties(&pr, dev));
+  std::strncpy(props->name, pr.name, sizeof(props->name) - 1);
+  std::strncpy(props->arch, pr.gcnArchName, sizeof(props->arch) - 1);
+  props->cu_count = pr.multiProcessorCount;
+  props->clock_khz = pr.clockRate;
+  int a = 0, b = 0;
+  (void)hipDeviceGetAttribute(&a, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  (void)hipDeviceGetAttribute(&b, hipDeviceAttributeSharedMemPerBlockOptin, dev);
+  props->lds_per_block = std::max(a, b);
+  props->lds_per_cu = (int)pr.maxSharedMemoryPerMultiProcessor;
+  props->l2_bytes = pr.l2CacheSize;
+  props->hbm_bytes = pr.totalGlobalMem;
+  return AF_OK;
 }
 
 af_status af_device_count(int *n) {

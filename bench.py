@@ -159,6 +159,7 @@ def main():
     adds = plan.info.adds
 
     res = {
+        "device": capi.device_props(0),
         "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,

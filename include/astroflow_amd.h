@@ -133,7 +133,20 @@ af_status af_execute(af_plan *plan, const void *d_in, void *d_out,
 af_status af_execute_host(af_plan *plan, const void *h_in, void *h_out,
                           uint8_t *h_masks);
 
+/* Properties of a device as the kernels see them. */
+typedef struct af_device_props {
+  char name[64];
+  char arch[32];           /* e.g. "gfx950" */
+  int cu_count;
+  int clock_khz;
+  int lds_per_block;       /* largest per-workgroup LDS the plans may use */
+  int lds_per_cu;
+  int l2_bytes;
+  size_t hbm_bytes;
+} af_device_props;
+
 /* Device helpers so callers need no other GPU runtime. */
+af_status af_device_query(int dev, af_device_props *props);
 af_status af_device_count(int *n);
 af_status af_set_device(int dev);
 af_status af_malloc(void **p, size_t bytes);

@@ -53,7 +53,8 @@ def _c_sizeof(type_name, tmp_path):
 
 @pytest.mark.parametrize("cname,pytype", [("af_plan_desc", capi.PlanDesc),
                                           ("af_plan_info", capi.PlanInfo),
-                                          ("af_stage_times", capi.StageTimes)])
+                                          ("af_stage_times", capi.StageTimes),
+                                          ("af_device_props", capi.DeviceProps)])
 def test_struct_layouts_match_c(cname, pytype, tmp_path):
     assert _c_sizeof(cname, tmp_path) == C.sizeof(pytype)
 
