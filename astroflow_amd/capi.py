@@ -88,6 +88,7 @@ SIGNATURES = {
     "af_free": (_I, [_P]),
     "af_memcpy_h2d": (_I, [_P, _P, _S, _P]),
     "af_memcpy_d2h": (_I, [_P, _P, _S, _P]),
+    "af_memcpy_d2d": (_I, [_P, _P, _S, _P]),
     "af_memset": (_I, [_P, _I, _S, _P]),
     "af_stream_create": (_I, [C.POINTER(_P)]),
     "af_stream_destroy": (_I, [_P]),

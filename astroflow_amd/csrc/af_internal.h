@@ -102,6 +102,13 @@ struct DdArgs {
   size_t nvalid;            // outputs written per row (direct: nout)
   int ntiles;               // time tiles of kDdTile
   int out16;                // 1: u16 output (u8 input, group width <= 256)
+  int lds_rows;             // u8 LDS-staged kernel: DM rows per workgroup (0 = off)
+  int lds_win;              // bytes of a channel's delay window staged in LDS
+  const int4 *worklist;     // optional (grp, tile, rowblk, -) per workgroup, in launch order
+  const int2 *chan_order;   // u8: [nrows][nch] (channel, delay), each group's channels
+                            // sorted by (delay mod 16) / 4; null = natural order
+  const int *qstart;        // [nrows][ngrp][4] first position of each quarter class
+  int nwork;
 };
 hipError_t launch_dedisperse(int dtype, const DdArgs &a, hipStream_t st);
 
@@ -133,6 +140,13 @@ struct af_plan {
   int nch = 0, ndm = 0, nsb = 0, ndm_nom = 0, maxdelay = 0, maxres = 0;
   int ntiles1 = 0;          // tiles of the first dedispersion stage
   int s2_tile = 0, s2_lds = 0, maxlds = 0;
+  int dd_lds_rows = 0, dd_lds_win = 0;   // LDS-staged u8 dedispersion
+  std::vector<int4> worklist;            // stage-1 workgroups in locality order
+  std::vector<int2> chan_order;          // (channel, delay) sorted by delay quarter class
+  std::vector<int> qstart;
+  int2 *d_chan_order = nullptr;
+  int *d_qstart = nullptr;
+  int4 *d_worklist = nullptr;
   double adds = 0;
   std::vector<int32_t> h_delays, h_res, h_maxres_j;
 
@@ -170,7 +184,7 @@ struct af_plan {
 
 namespace af {
 // 8-bit fast paths (af_kernels_u8.hip)
-constexpr int kStatRowsU8 = 256;   // rows per statistics item (16-bit lane sums)
+constexpr int kStatRowsU8 = 256;   // rows per statistics item (4 row lanes x 64)
 constexpr int kCleanRowsU8 = 32;   // rows per clean item (1024 threads x 8 rows x 16 ch)
 constexpr int kCleanU8MaxChan = 4096;
 hipError_t launch_seg_stats_u8(const uint8_t *x, int nchan, int c0, int c1, const RowItem *items,

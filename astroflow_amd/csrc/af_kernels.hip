@@ -430,13 +430,146 @@ __global__ __launch_bounds__(256) void k_clean(CleanArgs a) {
 // the odd bytes; after at most 256 channels the even-byte sums are
 // S - (H<<8).  That is two adds per four samples (three with the byte
 // extraction), exact because no 16-bit lane can exceed 256*255.
+// Bytes [s, s+16) of the 32 bytes lo|hi (s wave-uniform, so the switch is a
+// scalar branch): four v_alignbyte_b32.
+__device__ __forceinline__ uint32_t ab(uint32_t h, uint32_t l, uint32_t r) {
+  return __builtin_amdgcn_alignbyte(h, l, r);
+}
+
+__device__ __forceinline__ uint4 realign16(const uint4 lo, const uint4 hi, int s) {
+  const uint32_t r = (uint32_t)(s & 3);
+  switch (s >> 2) {
+  case 0: return make_uint4(ab(lo.y, lo.x, r), ab(lo.z, lo.y, r), ab(lo.w, lo.z, r), ab(hi.x, lo.w, r));
+  case 1: return make_uint4(ab(lo.z, lo.y, r), ab(lo.w, lo.z, r), ab(hi.x, lo.w, r), ab(hi.y, hi.x, r));
+  case 2: return make_uint4(ab(lo.w, lo.z, r), ab(hi.x, lo.w, r), ab(hi.y, hi.x, r), ab(hi.z, hi.y, r));
+  default: return make_uint4(ab(hi.x, lo.w, r), ab(hi.y, hi.x, r), ab(hi.z, hi.y, r), ab(hi.w, hi.z, r));
+  }
+}
+
+// Bytes [4q + r, 4q + r + 16) of lo|hi for a compile-time quarter q.
+template <int Q>
+__device__ __forceinline__ uint4 realign_q(const uint4 lo, const uint4 hi, uint32_t r) {
+  const uint32_t D[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+  return make_uint4(ab(D[Q + 1], D[Q], r), ab(D[Q + 2], D[Q + 1], r), ab(D[Q + 3], D[Q + 2], r),
+                    ab(D[Q + 4], D[Q + 3], r));
+}
+
+// Channels [i0, i1) of a row's sorted (channel, delay) list, all with delay
+// quarter class Q: two 16-byte aligned loads per lane (an aligned lane load
+// costs the L1 a fraction of a misaligned one) and a fixed realignment.
+template <int Q>
+__device__ __forceinline__ void dd_class(const uint8_t *__restrict__ tb, size_t Lp, uint32_t loff,
+                                         const int2 *__restrict__ cd, int i0, int i1,
+                                         uint32_t (&S)[4], uint32_t (&H)[4]) {
+#pragma unroll 8
+  for (int i = i0; i < i1; ++i) {
+    const int2 e = cd[i];
+    const uint8_t *rp = tb + (size_t)e.x * Lp + (e.y & ~15) + loff;
+    uint4 lo, hi;
+    __builtin_memcpy(&lo, __builtin_assume_aligned(rp, 16), 16);
+    __builtin_memcpy(&hi, __builtin_assume_aligned(rp + 16, 16), 16);
+    const uint4 w = realign_q<Q>(lo, hi, (uint32_t)(e.y & 3));
+    S[0] += w.x; S[1] += w.y; S[2] += w.z; S[3] += w.w;
+    H[0] += odd_bytes(w.x);
+    H[1] += odd_bytes(w.y);
+    H[2] += odd_bytes(w.z);
+    H[3] += odd_bytes(w.w);
+  }
+}
+
+// u8 dedispersion over quarter-class-sorted channel lists (same tiling and
+// packed-lane accumulation as k_dd_u8; channel blocks of <= 256 channels,
+// each summed in the four classes and flushed).
+__global__ __launch_bounds__(256) void k_dd_u8_q(DdArgs a) {
+  const int nrowblk = (a.nrows + kDdRowsPerBlock - 1) / kDdRowsPerBlock;
+  const uint32_t lid = xcd_remap(blockIdx.x, gridDim.x);
+  int rowblk, tile, grp;
+  if (a.worklist) {
+    const int4 wi = a.worklist[lid];
+    grp = wi.x;
+    tile = wi.y;
+    rowblk = wi.z;
+  } else {
+    rowblk = (int)(lid % (uint32_t)nrowblk);
+    const uint32_t rest = lid / (uint32_t)nrowblk;
+    tile = (int)(rest % (uint32_t)a.ntiles);
+    grp = (int)(rest / (uint32_t)a.ntiles);
+  }
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int row = rowblk * kDdRowsPerBlock + wave;
+  if (row >= a.nrows) return;
+  const int ca = grp * a.grp_w;
+  const int cb = min(ca + a.grp_w, a.nch);
+  const int blk_w = a.ngrp == 1 ? 256 : a.grp_w;
+  const int nblk_all = (a.nch + blk_w - 1) / blk_w;
+  const uint8_t *__restrict__ tb = (const uint8_t *)a.xT + (size_t)tile * kDdTile;
+  const uint32_t loff = (uint32_t)lane * kDdLanesOut;
+  const int2 *__restrict__ cd = a.chan_order + (size_t)row * a.nch;
+  uint32_t tot[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) tot[i] = 0;
+  for (int b0 = ca; b0 < cb; b0 += blk_w) {
+    const int blk = b0 / blk_w;
+    const int bend = min(b0 + blk_w, cb);
+    const int *qs = a.qstart + ((size_t)row * nblk_all + blk) * 4;
+    const int q0 = qs[0], q1 = qs[1], q2 = qs[2], q3 = qs[3];
+    uint32_t S[4] = {0, 0, 0, 0}, H[4] = {0, 0, 0, 0};
+    dd_class<0>(tb, a.Lp, loff, cd, q0, q1, S, H);
+    dd_class<1>(tb, a.Lp, loff, cd, q1, q2, S, H);
+    dd_class<2>(tb, a.Lp, loff, cd, q2, q3, S, H);
+    dd_class<3>(tb, a.Lp, loff, cd, q3, bend, S, H);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t L = S[k] - (H[k] << 8);
+      tot[4 * k + 0] += L & 0xFFFFu;
+      tot[4 * k + 1] += H[k] & 0xFFFFu;
+      tot[4 * k + 2] += L >> 16;
+      tot[4 * k + 3] += H[k] >> 16;
+    }
+  }
+  const size_t t = (size_t)tile * kDdTile + (size_t)lane * kDdLanesOut;
+  const size_t orow = ((size_t)row * a.ngrp + grp) * a.ostride;
+  if (a.out16) {
+    uint16_t *o = (uint16_t *)a.out + orow + t;
+    uint4 v0 = make_uint4(tot[0] | (tot[1] << 16), tot[2] | (tot[3] << 16),
+                          tot[4] | (tot[5] << 16), tot[6] | (tot[7] << 16));
+    uint4 v1 = make_uint4(tot[8] | (tot[9] << 16), tot[10] | (tot[11] << 16),
+                          tot[12] | (tot[13] << 16), tot[14] | (tot[15] << 16));
+    if (t + 16 <= a.nvalid) {
+      __builtin_memcpy(o, &v0, 16);
+      __builtin_memcpy(o + 8, &v1, 16);
+    } else {
+      for (int i = 0; i < 16; ++i)
+        if (t + i < a.nvalid) o[i] = (uint16_t)tot[i];
+    }
+  } else {
+    uint32_t *o = (uint32_t *)a.out + orow + t;
+    if (t + 16 <= a.nvalid) {
+      __builtin_memcpy(o, &tot[0], 64);
+    } else {
+      for (int i = 0; i < 16; ++i)
+        if (t + i < a.nvalid) o[i] = tot[i];
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_dd_u8(DdArgs a) {
   const int nrowblk = (a.nrows + kDdRowsPerBlock - 1) / kDdRowsPerBlock;
   const uint32_t lid = xcd_remap(blockIdx.x, gridDim.x);
-  const int rowblk = (int)(lid % (uint32_t)nrowblk);
-  const uint32_t rest = lid / (uint32_t)nrowblk;
-  const int tile = (int)(rest % (uint32_t)a.ntiles);
-  const int grp = (int)(rest / (uint32_t)a.ntiles);
+  int rowblk, tile, grp;
+  if (a.worklist) {
+    // host-sorted order: workgroups that read overlapping input run together
+    const int4 wi = a.worklist[lid];
+    grp = wi.x;
+    tile = wi.y;
+    rowblk = wi.z;
+  } else {
+    rowblk = (int)(lid % (uint32_t)nrowblk);
+    const uint32_t rest = lid / (uint32_t)nrowblk;
+    tile = (int)(rest % (uint32_t)a.ntiles);
+    grp = (int)(rest / (uint32_t)a.ntiles);
+  }
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int row = rowblk * kDdRowsPerBlock + wave;
@@ -456,10 +589,19 @@ __global__ __launch_bounds__(256) void k_dd_u8(DdArgs a) {
   for (int c0 = ca; c0 < cb; c0 += 256) {
     const int c1 = min(c0 + 256, cb);
     uint32_t S0 = 0, S1 = 0, S2 = 0, S3 = 0, H0 = 0, H1 = 0, H2 = 0, H3 = 0;
-#pragma unroll 4
+#pragma unroll 8
     for (int c = c0; c < c1; ++c) {
-      const uint8_t *rp = tbase + (size_t)c * a.Lp + drow[c];
-      const uint4 w = ldu4(rp + loff);
+      // Dword-aligned loads (16 bytes + the next dword) and a byte
+      // realignment by the wave-uniform delay remainder: a byte-misaligned
+      // 16-byte lane load costs the L1 about one access per lane.
+      const int d = drow[c];
+      const uint8_t *rp = tbase + (size_t)c * a.Lp + (d & ~3) + loff;
+      uint4 lo;
+      uint32_t nx;
+      __builtin_memcpy(&lo, __builtin_assume_aligned(rp, 4), 16);
+      __builtin_memcpy(&nx, __builtin_assume_aligned(rp + 16, 4), 4);
+      const uint32_t r = (uint32_t)(d & 3);
+      const uint4 w = make_uint4(ab(lo.y, lo.x, r), ab(lo.z, lo.y, r), ab(lo.w, lo.z, r), ab(nx, lo.w, r));
       S0 += w.x; S1 += w.y; S2 += w.z; S3 += w.w;
       H0 += odd_bytes(w.x);
       H1 += odd_bytes(w.y);
@@ -495,6 +637,132 @@ __global__ __launch_bounds__(256) void k_dd_u8(DdArgs a) {
     } else {
       for (int i = 0; i < 16; ++i)
         if (t + i < a.nvalid) o[i] = tot[i];
+    }
+  }
+}
+
+// u8 dedispersion with the delay windows staged in LDS.  A workgroup owns JB
+// consecutive DM rows (one per wave) and 1024 outputs of one channel group.
+// For each channel the union of the JB rows' windows -- from the first row's
+// delay (the smallest; delays grow with DM) to the last row's plus 1024 -- is
+// loaded once into LDS (16 bytes per thread, double-buffered, one barrier per
+// channel) and every wave reads its 16 bytes per lane from there.  The
+// packed-lane accumulation is that of k_dd_u8.
+template <int JB>
+__global__ __launch_bounds__(JB * 64) void k_dd_u8_lds(DdArgs a) {
+  extern __shared__ uint4 win_lds[];
+  const int nrowblk = (a.nrows + JB - 1) / JB;
+  const uint32_t lid = xcd_remap(blockIdx.x, gridDim.x);
+  int rowblk, tile, grp;
+  if (a.worklist) {
+    const int4 wi = a.worklist[lid];
+    grp = wi.x;
+    tile = wi.y;
+    rowblk = wi.z;
+  } else {
+    rowblk = (int)(lid % (uint32_t)nrowblk);
+    const uint32_t rest = lid / (uint32_t)nrowblk;
+    tile = (int)(rest % (uint32_t)a.ntiles);
+    grp = (int)(rest / (uint32_t)a.ntiles);
+  }
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int row0 = rowblk * JB;
+  const int row = row0 + wave;
+  const bool rvalid = row < a.nrows;
+  const int ca = grp * a.grp_w;
+  const int cb = min(ca + a.grp_w, a.nch);
+  const size_t t = (size_t)tile * kDdTile;
+  const uint8_t *__restrict__ tbase = (const uint8_t *)a.xT + t;
+  const int32_t *__restrict__ d0row = a.delays + (size_t)row0 * a.nch;
+  const int32_t *__restrict__ drow = a.delays + (size_t)(rvalid ? row : row0) * a.nch;
+  const int nchunk = a.lds_win >> 4;           // 16-byte chunks per window
+  const uint8_t *lbytes = (const uint8_t *)win_lds;
+  // Every thread loads and stores one 16-byte chunk per channel (threads past
+  // the window re-read its last chunk and write LDS nobody reads), and past
+  // the group's last channel the loads repeat that channel: no branch around
+  // the loads, so the wait before each LDS store covers only the oldest load.
+  const int chunk = min((int)threadIdx.x, nchunk - 1);
+  const uint8_t *gsrc = tbase + 16 * chunk;
+  const int wstride_all = blockDim.x;          // uint4 per LDS buffer
+  auto chan_src = [&](int x) {
+    const int xc = min(x, cb - 1);
+    // 16-byte aligned window start: aligned lane loads (an unaligned
+    // 16-byte lane load costs about one L1 access per lane)
+    return gsrc + (size_t)xc * a.Lp + (d0row[xc] & ~15);
+  };
+
+  // Register pipeline: channel x's window chunk lives in pre[(x - ca) % PD]
+  // from the load issued PD channels ahead until it is stored to LDS.
+  constexpr int PD = 4;
+  uint4 pre[PD];
+#pragma unroll
+  for (int k = 0; k < PD; ++k) pre[k] = ldu4(chan_src(ca + k));
+  win_lds[threadIdx.x] = pre[0];
+  pre[0] = ldu4(chan_src(ca + PD));
+  __syncthreads();
+
+  uint32_t tot[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) tot[i] = 0;
+  uint32_t S0 = 0, S1 = 0, S2 = 0, S3 = 0, H0 = 0, H1 = 0, H2 = 0, H3 = 0;
+  int since_flush = 0;
+  // The trip count is padded to a multiple of PD (padding channels only load
+  // and synchronise) so the unrolled body has no exit.
+  const int cpad = ca + (cb - ca + PD - 1) / PD * PD;
+  for (int cc = ca; cc < cpad; cc += PD) {
+#pragma unroll
+    for (int k = 0; k < PD; ++k) {
+      const int c = cc + k;
+      const int nb = (c - ca + 1) & 1;   // buffer of channel c+1
+      win_lds[nb * wstride_all + threadIdx.x] = pre[(k + 1) % PD];
+      pre[(k + 1) % PD] = ldu4(chan_src(c + 1 + PD));
+      if (rvalid && c < cb) {
+        const int off = drow[c] - (d0row[c] & ~15);
+        uint4 w;
+        __builtin_memcpy(&w, lbytes + (size_t)(nb ^ 1) * wstride_all * 16 + off + 16 * lane, 16);
+        S0 += w.x; S1 += w.y; S2 += w.z; S3 += w.w;
+        H0 += odd_bytes(w.x);
+        H1 += odd_bytes(w.y);
+        H2 += odd_bytes(w.z);
+        H3 += odd_bytes(w.w);
+      }
+      __syncthreads();
+      if (++since_flush == 256 || c + 1 == cb) {
+        since_flush = 0;
+        const uint32_t L0 = S0 - (H0 << 8), L1 = S1 - (H1 << 8);
+        const uint32_t L2 = S2 - (H2 << 8), L3 = S3 - (H3 << 8);
+        tot[0] += L0 & 0xFFFFu;  tot[1] += H0 & 0xFFFFu;  tot[2] += L0 >> 16;  tot[3] += H0 >> 16;
+        tot[4] += L1 & 0xFFFFu;  tot[5] += H1 & 0xFFFFu;  tot[6] += L1 >> 16;  tot[7] += H1 >> 16;
+        tot[8] += L2 & 0xFFFFu;  tot[9] += H2 & 0xFFFFu;  tot[10] += L2 >> 16; tot[11] += H2 >> 16;
+        tot[12] += L3 & 0xFFFFu; tot[13] += H3 & 0xFFFFu; tot[14] += L3 >> 16; tot[15] += H3 >> 16;
+        S0 = S1 = S2 = S3 = H0 = H1 = H2 = H3 = 0;
+      }
+    }
+  }
+  if (!rvalid) return;
+  const size_t to = t + (size_t)lane * kDdLanesOut;
+  const size_t orow = ((size_t)row * a.ngrp + grp) * a.ostride;
+  if (a.out16) {
+    uint16_t *o = (uint16_t *)a.out + orow + to;
+    uint4 v0 = make_uint4(tot[0] | (tot[1] << 16), tot[2] | (tot[3] << 16),
+                          tot[4] | (tot[5] << 16), tot[6] | (tot[7] << 16));
+    uint4 v1 = make_uint4(tot[8] | (tot[9] << 16), tot[10] | (tot[11] << 16),
+                          tot[12] | (tot[13] << 16), tot[14] | (tot[15] << 16));
+    if (to + 16 <= a.nvalid) {
+      __builtin_memcpy(o, &v0, 16);
+      __builtin_memcpy(o + 8, &v1, 16);
+    } else {
+      for (int i = 0; i < 16; ++i)
+        if (to + i < a.nvalid) o[i] = (uint16_t)tot[i];
+    }
+  } else {
+    uint32_t *o = (uint32_t *)a.out + orow + to;
+    if (to + 16 <= a.nvalid) {
+      __builtin_memcpy(o, &tot[0], 64);
+    } else {
+      for (int i = 0; i < 16; ++i)
+        if (to + i < a.nvalid) o[i] = tot[i];
     }
   }
 }
@@ -674,7 +942,13 @@ hipError_t launch_dedisperse(int dtype, const DdArgs &a, hipStream_t st) {
   const dim3 g((unsigned)nblk), b(256);
   switch (dtype) {
   case AF_U8:
-    if (a.out16 || a.ngrp == 1) {
+    if (a.lds_rows == 16 && (a.out16 || a.ngrp == 1)) {
+      const size_t nblk16 = (size_t)((a.nrows + 15) / 16) * a.ntiles * a.ngrp;
+      const size_t lds = 2 * (size_t)1024 * 16;   // two buffers of one 16-byte chunk per thread
+      hipLaunchKernelGGL(k_dd_u8_lds<16>, dim3((unsigned)nblk16), dim3(1024), lds, st, a);
+    } else if (a.chan_order && (a.out16 || a.ngrp == 1)) {
+      hipLaunchKernelGGL(k_dd_u8_q, g, b, 0, st, a);
+    } else if (a.out16 || a.ngrp == 1) {
       hipLaunchKernelGGL(k_dd_u8, g, b, 0, st, a);
     } else {
       hipLaunchKernelGGL((k_dd_generic<uint8_t, uint32_t, uint32_t>), g, b, 0, st, a);

@@ -35,29 +35,36 @@ __device__ __forceinline__ uint4 load_row16(const uint8_t *row, int cb, int lim)
 }
 
 // ------------------------------------------------------ IQRM statistics --
-// Per (row block <= 256 rows, channel): sum and sum of squares.  Sums are
-// accumulated as packed 16-bit lanes (exact: <= 256*255), squares with one
-// multiply-add per byte.
-__global__ __launch_bounds__(256) void k_seg_stats_u8(const uint8_t *__restrict__ x, int nchan,
-                                                      int c0, int c1,
-                                                      const RowItem *__restrict__ items,
-                                                      uint2 *__restrict__ part) {
+// Per (row block <= 1024 rows, channel): sum and sum of squares.  A 1024-
+// thread block covers 4096 channels (16 per thread) of every fourth row per
+// row lane (4 lanes, interleaved, so each wave reads 1 KB of one row per
+// load).  Per thread, sums are packed 16-bit lanes (exact: <= 256 rows x 255)
+// and squares one multiply-add per byte; the four row lanes are combined in
+// LDS.  Partials: u32 sum (<= 1024*255) and u32 square sum (<= 1024*65025).
+constexpr int kStatThreads = 1024;
+
+__global__ __launch_bounds__(1024) void k_seg_stats_u8(const uint8_t *__restrict__ x, int nchan,
+                                                       int c0, int c1,
+                                                       const RowItem *__restrict__ items,
+                                                       uint2 *__restrict__ part, int nunits) {
+  __shared__ uint32_t ls[4096], lq[4096];
   const int Csub = c1 - c0;
   const int ngrp = (Csub + 15) / 16;
   const int nchunk = (ngrp + 255) / 256;
-  const int item = blockIdx.x / nchunk;
-  const int g = (blockIdx.x % nchunk) * 256 + threadIdx.x;
-  if (g >= ngrp) return;
+  // persistent: a grid sized to the machine walks all (item, chunk) units
+  for (int unit = blockIdx.x; unit < nunits; unit += gridDim.x) {
+  const int item = unit / nchunk;
+  const int gl = threadIdx.x & 255;
+  const int rl = threadIdx.x >> 8;
+  const int g = (unit % nchunk) * 256 + gl;
+  const bool active = g < ngrp;
   const int cb = c0 + 16 * g;
   const RowItem it = items[item];
   uint32_t se[4] = {0, 0, 0, 0}, so[4] = {0, 0, 0, 0};
   uint32_t q[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) q[k] = 0;
-  const uint8_t *row = x + (size_t)it.r0 * nchan;
-#pragma unroll 2
-  for (uint32_t r = it.r0; r < it.r1; ++r, row += nchan) {
-    const uint4 w = load_row16(row, cb, c1);
+  auto accum = [&](const uint4 &w) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const uint32_t v = u4get(w, k);
@@ -69,29 +76,83 @@ __global__ __launch_bounds__(256) void k_seg_stats_u8(const uint8_t *__restrict_
         q[4 * k + b] += e * e;
       }
     }
+  };
+  if (active) {
+    const uint8_t *row = x + (size_t)(it.r0 + rl) * nchan;
+    const size_t step = (size_t)4 * nchan;
+    if (cb + 16 <= c1) {
+      // whole 16-channel groups: straight-line loads, several in flight
+      uint32_t r = it.r0 + rl;
+      for (; r + 12 < it.r1; r += 16, row += 4 * step) {
+        const uint4 w0 = ld16(row + cb), w1 = ld16(row + step + cb);
+        const uint4 w2 = ld16(row + 2 * step + cb), w3 = ld16(row + 3 * step + cb);
+        accum(w0);
+        accum(w1);
+        accum(w2);
+        accum(w3);
+      }
+      for (; r < it.r1; r += 4, row += step) accum(ld16(row + cb));
+    } else {
+      for (uint32_t r = it.r0 + rl; r < it.r1; r += 4, row += step) accum(load_row16(row, cb, c1));
+    }
   }
-  uint2 *o = part + (size_t)item * Csub + (cb - c0);
+  uint32_t s16[16];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const uint32_t s4[4] = {se[k] & 0xFFFFu, so[k] & 0xFFFFu, se[k] >> 16, so[k] >> 16};
+    s16[4 * k + 0] = se[k] & 0xFFFFu;
+    s16[4 * k + 1] = so[k] & 0xFFFFu;
+    s16[4 * k + 2] = se[k] >> 16;
+    s16[4 * k + 3] = so[k] >> 16;
+  }
+  for (int l = 0; l < 4; ++l) {
+    if (rl == l) {
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
-      if (cb + 4 * k + b < c1) o[4 * k + b] = make_uint2(s4[b], q[4 * k + b]);
+      for (int k = 0; k < 16; ++k) {
+        const int i = 16 * gl + k;
+        ls[i] = (l ? ls[i] : 0u) + s16[k];
+        lq[i] = (l ? lq[i] : 0u) + q[k];
+      }
+    }
+    __syncthreads();
+  }
+  if (rl == 0 && active) {
+    uint2 *o = part + (size_t)item * Csub + (cb - c0);
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+      if (cb + k < c1) o[k] = make_uint2(ls[16 * gl + k], lq[16 * gl + k]);
+  }
+  __syncthreads();   // LDS is reused by the next unit
   }
 }
 
-__global__ void k_seg_reduce_u8(const uint2 *__restrict__ part, const int *__restrict__ seg_items,
-                                int nseg, int Csub, Part *__restrict__ segsum) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)nseg * Csub) return;
-  const int s = (int)(idx / Csub), c = (int)(idx % Csub);
+// Sum the row-block partials of each segment: 64 channels x 16 item lanes
+// per block, combined in LDS.
+__global__ __launch_bounds__(1024) void k_seg_reduce_u8(const uint2 *__restrict__ part,
+                                                        const int *__restrict__ seg_items,
+                                                        int nseg, int Csub,
+                                                        Part *__restrict__ segsum) {
+  __shared__ uint64_t la[16][64], lb[16][64];
+  const int nchunk = (Csub + 63) / 64;
+  const int s = blockIdx.x / nchunk;
+  const int c = (blockIdx.x % nchunk) * 64 + (threadIdx.x & 63);
+  const int il = threadIdx.x >> 6;
   uint64_t a = 0, b = 0;
-  for (int i = seg_items[s]; i < seg_items[s + 1]; ++i) {
-    const uint2 p = part[(size_t)i * Csub + c];
-    a += p.x;
-    b += p.y;
+  if (c < Csub)
+    for (int i = seg_items[s] + il; i < seg_items[s + 1]; i += 16) {
+      const uint2 p = part[(size_t)i * Csub + c];
+      a += p.x;
+      b += p.y;
+    }
+  la[il][threadIdx.x & 63] = a;
+  lb[il][threadIdx.x & 63] = b;
+  __syncthreads();
+  if (il == 0 && c < Csub) {
+    for (int k = 1; k < 16; ++k) {
+      a += la[k][threadIdx.x];
+      b += lb[k][threadIdx.x];
+    }
+    segsum[(size_t)s * Csub + c] = Part{a, b, 0};
   }
-  segsum[idx] = Part{a, b, 0};
 }
 
 // ------------------------------------------------ clean + transpose (u8) --
@@ -135,35 +196,44 @@ __global__ __launch_bounds__(1024) void k_clean_u8(CleanArgs a) {
     s_cnt[threadIdx.x] = 0;
   }
   // per-thread channel masks (0xFF = keep / in range)
+  // The IQRM segment mask and the static mask are 0/1 bytes: one 16-byte
+  // load each, expanded to 0x00/0xFF lanes by a multiply (no carries).
   uint32_t km[4], zr[4], st[4];
   const uint8_t *smask = a.segmask ? a.segmask + (size_t)it.seg * a.nchan : nullptr;
+  uint4 sm16 = make_uint4(0, 0, 0, 0), st16 = make_uint4(0, 0, 0, 0);
+  if (smask) sm16 = load_row16(smask, cb, a.nchan);
+  if (a.stat_mask) st16 = load_row16(a.stat_mask, cb, a.nchan);
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    uint32_t m1 = 0, m2 = 0, m3 = 0;
+    uint32_t inside = 0, zin = 0;
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       const int c = cb + 4 * k + b;
-      const bool inside = c < a.nchan;
-      const bool keep = inside && !(smask && smask[c]);
-      const bool zin = a.zdm && c >= a.zc0 && c < a.zc1;
-      const bool sk = inside && !(a.stat_mask && a.stat_mask[c]);
-      m1 |= (keep ? 0xFFu : 0u) << (8 * b);
-      m2 |= (zin ? 0xFFu : 0u) << (8 * b);
-      m3 |= (sk ? 0xFFu : 0u) << (8 * b);
+      inside |= (c < a.nchan ? 0xFFu : 0u) << (8 * b);
+      zin |= ((a.zdm && c >= a.zc0 && c < a.zc1) ? 0xFFu : 0u) << (8 * b);
     }
-    km[k] = m1;
-    zr[k] = m2;
-    st[k] = m3;
+    km[k] = ~(u4get(sm16, k) * 0xFFu) & inside;
+    zr[k] = zin;
+    st[k] = ~(u4get(st16, k) * 0xFFu) & inside;
   }
   uint4 w[8];
+  {
+    const uint8_t *rowp = x + (size_t)(it.r0 + 8 * q) * a.nchan + cb;
+    if (cb + 16 <= a.nchan && 8 * q + 8 <= nrows) {
+      // all eight rows present: issue the eight loads back to back
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const int r = 8 * q + i;
-    w[i] = make_uint4(0, 0, 0, 0);
-    if (r < nrows && cb < a.nchan) {
-      uint4 v = load_row16(x + (size_t)(it.r0 + r) * a.nchan, cb, a.nchan);
-      w[i] = make_uint4(v.x & km[0], v.y & km[1], v.z & km[2], v.w & km[3]);
+      for (int i = 0; i < 8; ++i) w[i] = ld16(rowp + (size_t)i * a.nchan);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        w[i] = make_uint4(0, 0, 0, 0);
+        if (8 * q + i < nrows && cb < a.nchan)
+          w[i] = load_row16(x + (size_t)(it.r0 + 8 * q + i) * a.nchan, cb, a.nchan);
+      }
     }
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      w[i] = make_uint4(w[i].x & km[0], w[i].y & km[1], w[i].z & km[2], w[i].w & km[3]);
   }
   __syncthreads();
   if (a.zdm) {
@@ -308,15 +378,19 @@ hipError_t launch_seg_stats_u8(const uint8_t *x, int nchan, int c0, int c1, cons
                                int nitems, uint2 *part, hipStream_t st) {
   const int ngrp = (c1 - c0 + 15) / 16;
   const int nchunk = (ngrp + 255) / 256;
-  hipLaunchKernelGGL(k_seg_stats_u8, dim3((unsigned)(nitems * nchunk)), dim3(256), 0, st, x,
-                     nchan, c0, c1, items, part);
+  // one block per unit (the persistent form measured slower: its last
+  // round leaves CUs idle while a full grid overlaps block tails)
+  const int nunits = nitems * nchunk;
+  const int grid = nunits;
+  hipLaunchKernelGGL(k_seg_stats_u8, dim3((unsigned)grid), dim3(kStatThreads), 0, st,
+                     x, nchan, c0, c1, items, part, nunits);
   return hipGetLastError();
 }
 
 hipError_t launch_seg_reduce_u8(const uint2 *part, const int *seg_items, int nseg, int Csub,
                                 Part *segsum, hipStream_t st) {
-  const size_t n = (size_t)nseg * Csub;
-  hipLaunchKernelGGL(k_seg_reduce_u8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, part,
+  const int nchunk = (Csub + 63) / 64;
+  hipLaunchKernelGGL(k_seg_reduce_u8, dim3((unsigned)(nseg * nchunk)), dim3(1024), 0, st, part,
                      seg_items, nseg, Csub, segsum);
   return hipGetLastError();
 }

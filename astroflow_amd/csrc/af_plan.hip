@@ -268,9 +268,29 @@ static af_status build(af_plan *p) {
     p->adds = (double)p->ndm_nom * p->nch * (double)(p->nout + p->maxres) +
               (double)p->ndm * p->nsb * (double)p->nout;
   }
+  // LDS-staged u8 dedispersion: 16 DM rows per workgroup share each channel's
+  // delay window (span of the 16 rows' delays + one tile) if it fits 16 KB.
+  if (d.dtype == AF_U8 && (d.method == AF_METHOD_DIRECT || d.sb_nchan <= 256)) {
+    const int rows = d.method == AF_METHOD_DIRECT ? p->ndm : p->ndm_nom;
+    int span = 0;
+    for (int r0 = 0; r0 < rows; r0 += 16) {
+      const int r1 = std::min(r0 + 16, rows) - 1;
+      for (int c = 0; c < p->nch; ++c)
+        span = std::max(span, p->h_delays[(size_t)r1 * p->nch + c] - p->h_delays[(size_t)r0 * p->nch + c]);
+    }
+    // + up to 15 bytes of misalignment: windows start 16-byte aligned
+    const int win = (span + 15 + kDdTile + 15) / 16 * 16;
+    if (d.dm_per_block == 16 && win <= 16 * 1024 && rows >= 8) {
+      p->dd_lds_rows = 16;
+      p->dd_lds_win = win;
+    }
+  }
   p->ntiles1 = (int)((n_ext + kDdTile - 1) / kDdTile);
   p->L1 = (size_t)p->ntiles1 * kDdTile;
-  size_t need_len = p->L1 + (size_t)p->maxdelay + 64;
+  // Row length of the channel-major buffer: every load of the dedispersion
+  // kernels (tile start + delay + 16 bytes, or + the LDS window) stays inside
+  // its own channel row, which is zero beyond nbin.
+  size_t need_len = p->L1 + (size_t)p->maxdelay + 64 + (size_t)p->dd_lds_win;
   p->Lp = std::max(p->nbin, need_len);
   p->Lp = (p->Lp + 255) / 256 * 256;
 
@@ -340,7 +360,63 @@ static af_status build(af_plan *p) {
       p->clean_items.push_back(RowItem{(uint32_t)s, r, std::min<uint32_t>(r + clean_rows, p->segs[s].t1), 0});
   }
 
+  // ---- stage-1 work order.  Workgroup (group g, tile t, row block b) reads
+  // channel c from t*T + d(first row of b, c) onwards.  Sorting each group's
+  // workgroups by that start (at the group's middle channel) makes the
+  // workgroups that run together on one XCD (consecutive launch order after
+  // the XCD remap) read overlapping input, so it is fetched from HBM about
+  // once instead of once per tile.
+  if (d.dtype == AF_U8) {
+    const int rows = d.method == AF_METHOD_DIRECT ? p->ndm : p->ndm_nom;
+    const int JB = p->dd_lds_rows ? p->dd_lds_rows : kDdRowsPerBlock;
+    const int nrb = (rows + JB - 1) / JB;
+    const int ngrp = d.method == AF_METHOD_DIRECT ? 1 : p->nsb;
+    const int gw = d.method == AF_METHOD_DIRECT ? p->nch : d.sb_nchan;
+    std::vector<std::pair<int64_t, int4>> keyed;
+    keyed.reserve((size_t)ngrp * p->ntiles1 * nrb);
+    for (int g = 0; g < ngrp; ++g) {
+      const int cmid = std::min(p->nch - 1, g * gw + (std::min(gw, p->nch - g * gw) - 1) / 2);
+      for (int t = 0; t < p->ntiles1; ++t)
+        for (int b = 0; b < nrb; ++b) {
+          const int64_t start = (int64_t)t * kDdTile + p->h_delays[(size_t)(b * JB) * p->nch + cmid];
+          keyed.push_back({(int64_t)g << 40 | start, int4{g, t, b, 0}});
+        }
+    }
+    std::stable_sort(keyed.begin(), keyed.end(),
+                     [](const auto &x, const auto &y) { return x.first < y.first; });
+    p->worklist.reserve(keyed.size());
+    for (auto &kv : keyed) p->worklist.push_back(kv.second);
+  }
+
+  // ---- channel order for the aligned-load u8 kernel: per DM row and channel
+  // block (<= 256 channels), channels sorted by the quarter (delay mod 16)/4
+  // of their delay, with the start of each class.
+  if (d.dtype == AF_U8 && p->dd_lds_rows == 0 &&
+      (d.method == AF_METHOD_DIRECT || d.sb_nchan <= 256)) {
+    const int rows = d.method == AF_METHOD_DIRECT ? p->ndm : p->ndm_nom;
+    const int bw = d.method == AF_METHOD_DIRECT ? 256 : d.sb_nchan;
+    const int nblk = (p->nch + bw - 1) / bw;
+    p->chan_order.resize((size_t)rows * p->nch);
+    p->qstart.resize((size_t)rows * nblk * 4);
+    for (int r = 0; r < rows; ++r) {
+      const int32_t *dr = &p->h_delays[(size_t)r * p->nch];
+      int2 *out = &p->chan_order[(size_t)r * p->nch];
+      for (int b = 0; b < nblk; ++b) {
+        const int c0 = b * bw, c1 = std::min(c0 + bw, p->nch);
+        int pos = c0;
+        for (int q = 0; q < 4; ++q) {
+          p->qstart[((size_t)r * nblk + b) * 4 + q] = pos;
+          for (int c = c0; c < c1; ++c)
+            if (((dr[c] & 15) >> 2) == q) out[pos++] = int2{c, dr[c]};
+        }
+      }
+    }
+  }
+
   // ---- device memory
+  AF_TRY(upload(p, &p->d_worklist, p->worklist));
+  AF_TRY(upload(p, &p->d_chan_order, p->chan_order));
+  AF_TRY(upload(p, &p->d_qstart, p->qstart));
   AF_TRY(upload(p, &p->d_delays, p->h_delays));
   AF_TRY(upload(p, &p->d_res, p->h_res));
   AF_TRY(upload(p, &p->d_maxres_j, p->h_maxres_j));
@@ -463,6 +539,12 @@ static af_status execute(af_plan *p, const void *d_in, void *d_out, uint8_t *d_m
   da.delays = p->d_delays;
   da.nch = p->nch;
   da.ntiles = p->ntiles1;
+  da.lds_rows = p->dd_lds_rows;
+  da.lds_win = p->dd_lds_win;
+  da.worklist = p->d_worklist;
+  da.chan_order = p->d_chan_order;
+  da.qstart = p->d_qstart;
+  da.nwork = (int)p->worklist.size();
   if (d.method == AF_METHOD_DIRECT) {
     da.nrows = p->ndm;
     da.ngrp = 1;
@@ -724,6 +806,11 @@ af_status af_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream) 
 af_status af_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream) {
   AF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
   AF_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return AF_OK;
+}
+
+af_status af_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream) {
+  AF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
   return AF_OK;
 }
 

@@ -39,8 +39,13 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--dm-per-block", type=int, default=0,
+                    help="DM rows per stage-1 workgroup (0 = library default)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="target CPU work for the baseline sample")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one 256 MiB device-to-device copy (a known-size kernel "
+                         "for scaling hardware traffic counters)")
     return ap.parse_args()
 
 
@@ -120,11 +125,18 @@ def main():
     capi.check(capi.lib().af_set_device(0))
     threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)))
     x, spec, info = make_workload(threads)
+    spec.dm_per_block = a.dm_per_block
     plan = capi.Plan(spec)
     d_in = capi.DeviceBuffer(x.nbytes)
     d_in.upload(x)
     d_out = capi.DeviceBuffer(info["ndm"] * info["nout"] * 4)
 
+    if a.calib_copy:
+        cb = capi.DeviceBuffer(256 << 20)
+        capi.check(capi.lib().af_memcpy_d2d(capi.C.c_void_p(cb.ptr), capi.C.c_void_p(d_in.ptr),
+                                            256 << 20, None))
+        capi.sync()
+        cb.free()
     for _ in range(a.warmup):
         plan.execute(d_in.ptr, d_out.ptr)
     capi.sync()

@@ -153,6 +153,7 @@ af_status af_malloc(void **p, size_t bytes);
 af_status af_free(void *p);
 af_status af_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
 af_status af_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+af_status af_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
 af_status af_memset(void *dst, int value, size_t bytes, void *stream);
 af_status af_stream_create(void **stream);
 af_status af_stream_destroy(void *stream);
