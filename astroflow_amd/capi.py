@@ -52,7 +52,7 @@ class PlanInfo(C.Structure):
     _fields_ = [("nbin", C.c_size_t), ("nout", C.c_size_t), ("ndm", C.c_int),
                 ("out_elem_bytes", C.c_int), ("nwin", C.c_int), ("nsb", C.c_int),
                 ("ndm_nom", C.c_int), ("max_delay", C.c_int), ("scratch_bytes", C.c_size_t),
-                ("adds", C.c_double)]
+                ("adds", C.c_double), ("inter_bytes", C.c_size_t)]
 
 
 class DeviceProps(C.Structure):

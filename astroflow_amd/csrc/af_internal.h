@@ -139,6 +139,7 @@ struct af_plan {
   size_t nbin = 0, nout = 0, Lp = 0, L1 = 0;
   int nch = 0, ndm = 0, nsb = 0, ndm_nom = 0, maxdelay = 0, maxres = 0;
   int ntiles1 = 0;          // tiles of the first dedispersion stage
+  int inter_elem = 0;       // bytes per subband partial sum
   int s2_tile = 0, s2_lds = 0, maxlds = 0;
   int dd_lds_rows = 0, dd_lds_win = 0;   // LDS-staged u8 dedispersion
   std::vector<int4> worklist;            // stage-1 workgroups in locality order

@@ -440,8 +440,8 @@ static af_status build(af_plan *p) {
   AF_TRY(dalloc(p, &p->d_xT, (size_t)p->nch * p->Lp * es));
   AF_HIP(hipMemset(p->d_xT, 0, (size_t)p->nch * p->Lp * es));
   if (d.method == AF_METHOD_SUBBAND) {
-    size_t ies = (d.dtype == AF_U8 && d.sb_nchan <= 256) ? 2 : (d.dtype == AF_U32 || d.dtype == AF_F32 ? 8 : 4);
-    AF_TRY(dalloc(p, &p->d_inter, (size_t)p->ndm_nom * p->nsb * p->L1 * ies));
+    p->inter_elem = (d.dtype == AF_U8 && d.sb_nchan <= 256) ? 2 : (d.dtype == AF_U32 || d.dtype == AF_F32 ? 8 : 4);
+    AF_TRY(dalloc(p, &p->d_inter, (size_t)p->ndm_nom * p->nsb * p->L1 * p->inter_elem));
   }
   if (d.use_iqrm && p->nwin > 0) {
     void *q;
@@ -641,6 +641,7 @@ af_status af_plan_query(const af_plan *p, af_plan_info *info) {
   info->max_delay = p->maxdelay;
   info->scratch_bytes = p->scratch_bytes;
   info->adds = p->adds;
+  info->inter_bytes = p->nsb > 0 ? (size_t)p->ndm_nom * p->nsb * p->L1 * p->inter_elem : 0;
   return AF_OK;
 }
 

@@ -31,6 +31,23 @@ METRIC = "DM-trial×samples/s at 4096ch uint8; p99 chunk latency, batched/served
 UNIT = "DM-trial*samples/s"
 HBM_PEAK_GBS = 8000.0
 VALU_ADD_PEAK = 256 * 64 * 2.4e9     # 32-bit adds/s (one per lane per clock)
+STAGE1_KERNEL = "k_dd_u8_q"
+# FETCH_SIZE/WRITE_SIZE summary of this workload, scaled by the copy-kernel
+# calibration; one "traffic_bytes <kernel> <bytes>" line per kernel.
+TRAFFIC_FILE = "profiles/hbm_traffic_4096ch_u8_subband_iqrm_zdm.md"
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprof summary."""
+    path = os.path.join(ROOT, TRAFFIC_FILE)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes" and parts[1] == kernel:
+                return float(parts[2])
+    return None
 
 
 def parse():
@@ -166,9 +183,13 @@ def main():
     b_in = info["nchan"] * (info["nout"] * info["ds"] + info["dmax"]) * 1
     b_out = info["ndm"] * info["nout"] * 4
     b_alg = b_in + b_out
+    # dominant kernel: subband stage 1.  Its algorithmic bytes are the cleaned
+    # input window read once plus the subband partial sums written once.
     dom_ms = stages["dedisp"]
-    achieved = b_alg / (dom_ms / 1e3) / 1e9
+    b_stage1 = b_in + plan.info.inter_bytes
+    achieved = b_stage1 / (dom_ms / 1e3) / 1e9
     adds = plan.info.adds
+    traffic = profiled_traffic(STAGE1_KERNEL)
 
     res = {
         "device": capi.device_props(0),
@@ -183,9 +204,10 @@ def main():
                    "nchan": info["nchan"], "nout": info["nout"], "nsamp_in": info["nsamp"],
                    "ndm": info["ndm"], "nsub": 32, "ndm0": 32, "path": "plain single call"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_dd_u8 (subband stage 1)",
-                     "kernel_ms": dom_ms, "alg_bytes": b_alg},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": STAGE1_KERNEL + " (subband stage 1)",
+                     "kernel_ms": dom_ms, "alg_bytes": b_stage1,
+                     "traffic_source": TRAFFIC_FILE if traffic else None},
         "path_roofline": {"alg_bytes": b_alg, "GB_s": b_alg / (ms / 1e3) / 1e9,
                           "hbm_frac": b_alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
                           "adds": adds, "valu_add_frac": adds / (ms / 1e3) / VALU_ADD_PEAK},

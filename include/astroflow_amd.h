@@ -97,6 +97,7 @@ typedef struct af_plan_info {
   int max_delay;           /* largest delay in binned samples over all DMs      */
   size_t scratch_bytes;    /* device memory held by the plan                    */
   double adds;             /* integer/float adds the method performs per call   */
+  size_t inter_bytes;      /* subband intermediate written by stage 1 per call  */
 } af_plan_info;
 
 /* Per-stage device time, averaged over the n executes run since profiling
