@@ -67,6 +67,28 @@ class StageTimes(C.Structure):
                 ("total", C.c_float), ("n", C.c_int)]
 
 
+class ServerDesc(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_inflight", C.c_int), ("max_queue", C.c_int)]
+
+
+class ServerResult(C.Structure):
+    _fields_ = [("beam", C.c_int), ("has_plane", C.c_int), ("status", C.c_int),
+                ("tag", C.c_uint64), ("window", C.c_uint64),
+                ("d_plane", C.c_void_p), ("d_masks", C.c_void_p),
+                ("t_submit_ms", C.c_double), ("t_admit_ms", C.c_double),
+                ("t_done_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("admitted", C.c_uint64),
+                ("completed", C.c_uint64), ("planes", C.c_uint64),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double),
+                ("mean_ms", C.c_double), ("first_submit_ms", C.c_double),
+                ("last_done_ms", C.c_double), ("hbm_used", C.c_size_t),
+                ("hbm_peak", C.c_size_t), ("hbm_budget", C.c_size_t),
+                ("inflight_peak", C.c_int)]
+
+
 _lib = None
 
 # name -> (restype, argtypes); every symbol declared in include/astroflow_amd.h
@@ -95,6 +117,16 @@ SIGNATURES = {
     "af_stream_sync": (_I, [_P]),
     "af_sync": (_I, []),
     "af_time_execute": (_I, [_P, _P, _P, _I, C.POINTER(C.c_float)]),
+    "af_host_alloc": (_I, [C.POINTER(_P), _S]),
+    "af_host_free": (_I, [_P]),
+    "af_server_create": (_I, [C.POINTER(ServerDesc), C.POINTER(_P)]),
+    "af_server_destroy": (_I, [_P]),
+    "af_server_add_beam": (_I, [_P, C.POINTER(PlanDesc), _S, C.POINTER(_I)]),
+    "af_server_submit": (_I, [_P, _I, _P, C.c_uint64]),
+    "af_server_poll": (_I, [_P, C.POINTER(ServerResult), _I, C.POINTER(_I), _I]),
+    "af_server_release": (_I, [_P, _P]),
+    "af_server_drain": (_I, [_P]),
+    "af_server_get_stats": (_I, [_P, C.POINTER(ServerStats)]),
     "af_last_error": (C.c_char_p, []),
     "af_version": (C.c_char_p, []),
 }
@@ -330,3 +362,78 @@ class DeviceBuffer:
 
 def sync():
     check(lib().af_sync(), "af_sync")
+
+
+class HostBuffer:
+    """Page-locked host memory (af_host_alloc) viewed as a numpy array."""
+
+    def __init__(self, shape, dtype):
+        dtype = np.dtype(dtype)
+        n = int(np.prod(shape)) * dtype.itemsize
+        p = C.c_void_p()
+        check(lib().af_host_alloc(C.byref(p), n), "af_host_alloc")
+        self.ptr = p.value
+        buf = (C.c_char * n).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=dtype).reshape(shape)
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            lib().af_host_free(C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Server:
+    """Owns an af_server: resident per-beam windows, budgeted admission."""
+
+    def __init__(self, hbm_budget: int = 0, max_inflight: int = 0, max_queue: int = 0):
+        d = ServerDesc(hbm_budget, max_inflight, max_queue)
+        h = C.c_void_p()
+        check(lib().af_server_create(C.byref(d), C.byref(h)), "af_server_create")
+        self._h = h
+        self.beams = []
+
+    def add_beam(self, window: DedispSpec, chunk_rows: int) -> int:
+        desc, keep = window.to_desc()
+        b = C.c_int(-1)
+        check(lib().af_server_add_beam(self._h, C.byref(desc), chunk_rows, C.byref(b)),
+              "af_server_add_beam")
+        self.beams.append((window, chunk_rows))
+        return b.value
+
+    def submit(self, beam: int, rows_ptr: int, tag: int = 0):
+        check(lib().af_server_submit(self._h, beam, C.c_void_p(rows_ptr), tag), "af_server_submit")
+
+    def poll(self, max_n: int = 64, wait_ms: int = 0):
+        arr = (ServerResult * max_n)()
+        n = C.c_int(0)
+        check(lib().af_server_poll(self._h, arr, max_n, C.byref(n), wait_ms), "af_server_poll")
+        return [arr[i] for i in range(n.value)]
+
+    def release(self, d_plane: int):
+        check(lib().af_server_release(self._h, C.c_void_p(d_plane)), "af_server_release")
+
+    def drain(self):
+        check(lib().af_server_drain(self._h), "af_server_drain")
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        check(lib().af_server_get_stats(self._h, C.byref(st)), "af_server_get_stats")
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().af_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

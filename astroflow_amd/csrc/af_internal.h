@@ -43,6 +43,7 @@ int acc_size(int dtype);  // bytes of one output element
 
 // Error state (thread-local message).
 void set_error(const std::string &msg);
+std::string error_text();  // the calling thread's last message
 af_status fail(af_status code, const std::string &msg);
 
 #define AF_HIP(call)                                                          \
@@ -51,6 +52,12 @@ af_status fail(af_status code, const std::string &msg);
     if (e_ != hipSuccess)                                                     \
       return ::af::fail(AF_ERR_RUNTIME, std::string(#call) + ": " +           \
                                             hipGetErrorString(e_));           \
+  } while (0)
+
+#define AF_TRY(x)                                                             \
+  do {                                                                        \
+    af_status s_ = (x);                                                       \
+    if (s_ != AF_OK) return s_;                                               \
   } while (0)
 
 // ----------------------------------------------------------------------
@@ -184,6 +191,9 @@ struct af_plan {
 };
 
 namespace af {
+// Runs every stage of one plan on `st` (af_plan.hip).
+af_status execute(af_plan *p, const void *d_in, void *d_out, uint8_t *d_masks, hipStream_t st);
+
 // 8-bit fast paths (af_kernels_u8.hip)
 constexpr int kStatRowsU8 = 256;   // rows per statistics item (4 row lanes x 64)
 constexpr int kCleanRowsU8 = 32;   // rows per clean item (1024 threads x 8 rows x 16 ch)

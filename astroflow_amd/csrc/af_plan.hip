@@ -16,6 +16,7 @@ namespace af {
 static thread_local std::string g_err;
 
 void set_error(const std::string &msg) { g_err = msg; }
+std::string error_text() { return g_err; }
 
 af_status fail(af_status code, const std::string &msg) {
   g_err = msg;
@@ -129,12 +130,6 @@ static void release(af_plan *p) {
   if (p->stream) (void)hipStreamDestroy(p->stream);
   p->stream = nullptr;
 }
-
-#define AF_TRY(x)                                                             \
-  do {                                                                        \
-    af_status s_ = (x);                                                       \
-    if (s_ != AF_OK) return s_;                                               \
-  } while (0)
 
 static af_status validate(const af_plan_desc &d) {
   if (d.dtype < AF_U8 || d.dtype > AF_F32) return fail(AF_ERR_INVALID, "unknown dtype");
@@ -484,8 +479,7 @@ static af_status collect(af_plan *p) {
   return AF_OK;
 }
 
-static af_status execute(af_plan *p, const void *d_in, void *d_out, uint8_t *d_masks,
-                         hipStream_t st) {
+af_status execute(af_plan *p, const void *d_in, void *d_out, uint8_t *d_masks, hipStream_t st) {
   const af_plan_desc &d = p->d;
   int set = -1;
   if (p->profiling) {
@@ -795,6 +789,18 @@ af_status af_malloc(void **p, size_t bytes) {
 
 af_status af_free(void *p) {
   if (p) AF_HIP(hipFree(p));
+  return AF_OK;
+}
+
+af_status af_host_alloc(void **p, size_t bytes) {
+  if (!p) return fail(AF_ERR_INVALID, "null argument");
+  hipError_t e = hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault);
+  if (e != hipSuccess) return fail(AF_ERR_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+  return AF_OK;
+}
+
+af_status af_host_free(void *p) {
+  if (p) AF_HIP(hipHostFree(p));
   return AF_OK;
 }
 

@@ -165,6 +165,78 @@ af_status af_sync(void);
 af_status af_time_execute(af_plan *plan, const void *d_in, void *d_out, int reps,
                           float *ms);
 
+/* Page-locked host memory (hipHostMalloc), for asynchronous uploads. */
+af_status af_host_alloc(void **p, size_t bytes);
+af_status af_host_free(void *p);
+
+/* -------------------------------------------------------------- server --
+ * Real-time chunk server.  Replaces the reference's per-chunk call loop
+ * (python/astroflow/search.py:365,464 calling _dedisperse_spec on chunk +
+ * DM-overlap tail read from the file each time, src/core/dedispered.cu:1040
+ * uploading the whole window per call) with resident state:
+ *  - each beam keeps its DM-overlap carry (the last window - chunk rows) in
+ *    HBM, so only the chunk's new rows cross PCIe;
+ *  - a server thread admits submitted chunks while the HBM budget (beam
+ *    buffers + result planes not yet released) and the in-flight limit allow,
+ *    dispatches them on per-beam streams so that work of different beams
+ *    overlaps on the device, and timestamps completion;
+ *  - every result equals a plain af_execute of the beam's plan on the same
+ *    window of rows. */
+typedef struct af_server af_server;
+
+typedef struct af_server_desc {
+  size_t hbm_budget;       /* bytes for beam buffers + result planes; 0 = 90% of free memory */
+  int max_inflight;        /* chunks dispatched and not finished (0 = 16)              */
+  int max_queue;           /* submitted chunks waiting for admission (0 = 4096)        */
+} af_server_desc;
+
+/* One finished chunk.  Chunks submitted before a beam's first window is full
+ * only extend the carry: they finish with has_plane = 0. */
+typedef struct af_server_result {
+  int beam;
+  int has_plane;
+  int status;              /* AF_OK or the error of this chunk                          */
+  uint64_t tag;            /* caller's tag from af_server_submit                        */
+  uint64_t window;         /* output chunk index j: the window starts at beam row j*chunk */
+  void *d_plane;           /* [ndm][nout] result on the device; valid until released    */
+  uint8_t *d_masks;        /* [nwin][nchan] IQRM masks on the device (or NULL)          */
+  double t_submit_ms;      /* server clock: submit, admission, completion               */
+  double t_admit_ms;
+  double t_done_ms;
+} af_server_result;
+
+typedef struct af_server_stats {
+  uint64_t submitted, admitted, completed, planes;
+  double p50_ms, p99_ms, max_ms, mean_ms;   /* submit -> result ready, chunks with a plane */
+  double first_submit_ms, last_done_ms;     /* server clock                               */
+  size_t hbm_used, hbm_peak, hbm_budget;
+  int inflight_peak;
+} af_server_stats;
+
+af_status af_server_create(const af_server_desc *desc, af_server **srv);
+/* Stops the server thread after the work already submitted and frees everything. */
+af_status af_server_destroy(af_server *srv);
+/* Adds a beam.  window->nsamp is the window length W = chunk + DM-overlap
+ * tail rows; window->nout is the chunk's output length (ds must be 1).
+ * chunk_rows new rows arrive per chunk (chunk_rows = window->nout).  The
+ * beam's buffers count against the budget (AF_ERR_NOMEM when they do not
+ * fit).  Not allowed while chunks are in flight. */
+af_status af_server_add_beam(af_server *srv, const af_plan_desc *window, size_t chunk_rows,
+                             int *beam);
+/* Queues the beam's next chunk_rows rows ([chunk_rows][nchan], time-major).
+ * The rows are borrowed, not copied: keep them unchanged until this chunk's
+ * result comes back from af_server_poll (page-locked memory from
+ * af_host_alloc makes the upload asynchronous).  AF_ERR_BUSY: queue full. */
+af_status af_server_submit(af_server *srv, int beam, const void *h_rows, uint64_t tag);
+/* Returns up to max finished chunks, in completion order; waits up to
+ * wait_ms for the first one. */
+af_status af_server_poll(af_server *srv, af_server_result *out, int max, int *n, int wait_ms);
+/* Gives a result plane (and its masks) back to the pool. */
+af_status af_server_release(af_server *srv, void *d_plane);
+/* Waits until every submitted chunk has finished. */
+af_status af_server_drain(af_server *srv);
+af_status af_server_get_stats(af_server *srv, af_server_stats *st);
+
 const char *af_last_error(void);
 const char *af_version(void);
 
