@@ -68,7 +68,8 @@ class StageTimes(C.Structure):
 
 
 class ServerDesc(C.Structure):
-    _fields_ = [("hbm_budget", C.c_size_t), ("max_inflight", C.c_int), ("max_queue", C.c_int)]
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_inflight", C.c_int), ("max_queue", C.c_int),
+                ("full_rfi", C.c_int)]
 
 
 class ServerResult(C.Structure):
@@ -392,8 +393,9 @@ class HostBuffer:
 class Server:
     """Owns an af_server: resident per-beam windows, budgeted admission."""
 
-    def __init__(self, hbm_budget: int = 0, max_inflight: int = 0, max_queue: int = 0):
-        d = ServerDesc(hbm_budget, max_inflight, max_queue)
+    def __init__(self, hbm_budget: int = 0, max_inflight: int = 0, max_queue: int = 0,
+                 full_rfi: bool = False):
+        d = ServerDesc(hbm_budget, max_inflight, max_queue, int(full_rfi))
         h = C.c_void_p()
         check(lib().af_server_create(C.byref(d), C.byref(h)), "af_server_create")
         self._h = h

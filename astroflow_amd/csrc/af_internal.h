@@ -93,6 +93,7 @@ struct CleanArgs {
   const uint8_t *stat_mask; // [nchan] or null
   int zdm, zc0, zc1;
   int dd_c0, dd_c1;
+  const int *run_if;        // u8 kernel: skip the launch when *run_if == 0 (null = always run)
 };
 hipError_t launch_clean(int dtype, const CleanArgs &a, hipStream_t st);
 
@@ -204,4 +205,25 @@ hipError_t launch_seg_reduce_u8(const uint2 *part, const int *seg_items, int nse
                                 Part *segsum, hipStream_t st);
 hipError_t launch_clean_u8(const CleanArgs &a, hipStream_t st);
 hipError_t launch_stage2_u16v(const Stage2Args &a, hipStream_t st);
+
+// Served-beam helpers (af_kernels_u8.hip).
+// ring: [R][2][Csub] per-chunk sums (head rows, remaining rows).  out[c] =
+// sum over chunks first..first+nfull-1 (mod R) of head+rest, plus chunk
+// `last`'s head (and its rest when last_full).
+hipError_t launch_ring_sum(const Part *ring, int R, int first, int nfull, int last, int last_full,
+                           int Csub, Part *out, hipStream_t st);
+// flags[1] = (flags[0] == 0 || mask != prev); prev = mask; flags[0] = 1.
+hipError_t launch_mask_delta(const uint8_t *mask, uint8_t *prev, int n, int *flags, hipStream_t st);
+// Copies each of nrows rows' first width bytes, rounded up to 16 (bases and
+// pitch 16-byte aligned; the rounding may copy up to 15 bytes more per row).
+hipError_t launch_copy_rows(uint8_t *dst, const uint8_t *src, size_t pitch, size_t width,
+                            int nrows, hipStream_t st);
+// Zeroes width bytes of each of nrows rows spaced pitch bytes apart.
+hipError_t launch_zero_rows(uint8_t *base, size_t pitch, size_t width, int nrows, hipStream_t st);
+
+// Pieces of the plan's execute (af_plan.hip), for callers that keep their
+// own cleaned data: IQRM vote from p->d_segsum into p->d_masks / d_segmask,
+// and both dedispersion stages from a channel-major cleaned buffer.
+af_status run_iqrm_vote(af_plan *p, hipStream_t st);
+af_status run_dedisp(af_plan *p, const void *xT, size_t Lp, void *d_out, hipStream_t st, int set);
 }  // namespace af

@@ -183,6 +183,7 @@ __device__ __forceinline__ uint32_t nonzero_bytes(uint32_t w) {
 __global__ __launch_bounds__(1024) void k_clean_u8(CleanArgs a) {
   __shared__ uint32_t s_sum[32], s_cnt[32];
   __shared__ float s_m[32];
+  if (a.run_if && *a.run_if == 0) return;
   const uint32_t bid = xcd_remap_u8(blockIdx.x, gridDim.x);
   const RowItem it = a.items[bid];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -372,7 +373,104 @@ __global__ __launch_bounds__(512) void k_stage2_u16v(Stage2Args a) {
   }
 }
 
+// ------------------------------------------------ served-beam helpers --
+// Window statistics of a served beam from its ring of per-chunk sums: one
+// thread per channel, exact 64/128-bit integer adds.
+__global__ __launch_bounds__(256) void k_ring_sum(const Part *__restrict__ ring, int R, int first,
+                                                  int nfull, int last, int last_full, int Csub,
+                                                  Part *__restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= Csub) return;
+  uint64_t s = 0, lo = 0, hi = 0;
+  auto add = [&](const Part &p) {
+    s += p.s;
+    const uint64_t n = lo + p.q_lo;
+    hi += p.q_hi + (n < lo ? 1u : 0u);
+    lo = n;
+  };
+  for (int i = 0; i < nfull; ++i) {
+    const size_t sl = (size_t)((first + i) % R);
+    add(ring[(sl * 2 + 0) * Csub + c]);
+    add(ring[(sl * 2 + 1) * Csub + c]);
+  }
+  add(ring[((size_t)last * 2 + 0) * Csub + c]);
+  if (last_full) add(ring[((size_t)last * 2 + 1) * Csub + c]);
+  out[c] = Part{s, lo, hi};
+}
+
+// One workgroup: did the window mask change since the last call?
+__global__ __launch_bounds__(1024) void k_mask_delta(const uint8_t *__restrict__ mask,
+                                                     uint8_t *__restrict__ prev, int n,
+                                                     int *__restrict__ flags) {
+  int diff = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const uint8_t m = mask[i];
+    diff |= m != prev[i];
+    prev[i] = m;
+  }
+  diff = __syncthreads_or(diff);
+  if (threadIdx.x == 0) {
+    flags[1] = (diff || flags[0] == 0) ? 1 : 0;
+    flags[0] = 1;
+  }
+}
+
+// One workgroup row per (row, 4096-byte span); byte stores (the span start
+// is not aligned).
+__global__ __launch_bounds__(256) void k_zero_rows(uint8_t *__restrict__ base, size_t pitch,
+                                                   uint32_t width) {
+  uint8_t *row = base + (size_t)blockIdx.y * pitch;
+  for (uint32_t i = blockIdx.x * 4096u + threadIdx.x; i < min(width, blockIdx.x * 4096u + 4096u);
+       i += 256u)
+    row[i] = 0;
+}
+
+// Copies `width16` 16-byte units of each of nrows rows (row pitch `pitch`
+// bytes, both bases 16-byte aligned, pitch a multiple of 16).
+__global__ __launch_bounds__(256) void k_copy_rows16(uint4 *__restrict__ dst,
+                                                     const uint4 *__restrict__ src,
+                                                     size_t pitch16, uint32_t width16) {
+  const size_t row = (size_t)blockIdx.y * pitch16;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < width16; i += gridDim.x * 256u)
+    dst[row + i] = src[row + i];
+}
+
 // ---------------------------------------------------------- launchers --
+
+hipError_t launch_copy_rows(uint8_t *dst, const uint8_t *src, size_t pitch, size_t width,
+                            int nrows, hipStream_t st) {
+  if (width == 0 || nrows <= 0) return hipSuccess;
+  if (((uintptr_t)dst | (uintptr_t)src | pitch) & 15) return hipErrorInvalidValue;
+  const size_t w16 = (width + 15) / 16;
+  if (w16 > 0xFFFFFFFFull || nrows > 65535) return hipErrorInvalidValue;
+  unsigned gx = (unsigned)((w16 + 255) / 256);
+  if (gx > 64) gx = 64;
+  hipLaunchKernelGGL(k_copy_rows16, dim3(gx, (unsigned)nrows), dim3(256), 0, st, (uint4 *)dst,
+                     (const uint4 *)src, pitch / 16, (uint32_t)w16);
+  return hipGetLastError();
+}
+
+hipError_t launch_ring_sum(const Part *ring, int R, int first, int nfull, int last, int last_full,
+                           int Csub, Part *out, hipStream_t st) {
+  if (Csub <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ring_sum, dim3((unsigned)((Csub + 255) / 256)), dim3(256), 0, st, ring, R,
+                     first, nfull, last, last_full, Csub, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_mask_delta(const uint8_t *mask, uint8_t *prev, int n, int *flags, hipStream_t st) {
+  hipLaunchKernelGGL(k_mask_delta, dim3(1), dim3(1024), 0, st, mask, prev, n, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_rows(uint8_t *base, size_t pitch, size_t width, int nrows, hipStream_t st) {
+  if (width == 0 || nrows <= 0) return hipSuccess;
+  if (width > 0xFFFFFFFFull || nrows > 65535) return hipErrorInvalidValue;
+  const unsigned gx = (unsigned)((width + 4095) / 4096);
+  hipLaunchKernelGGL(k_zero_rows, dim3(gx, (unsigned)nrows), dim3(256), 0, st, base, pitch,
+                     (uint32_t)width);
+  return hipGetLastError();
+}
 
 hipError_t launch_seg_stats_u8(const uint8_t *x, int nchan, int c0, int c1, const RowItem *items,
                                int nitems, uint2 *part, hipStream_t st) {

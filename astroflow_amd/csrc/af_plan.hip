@@ -500,16 +500,7 @@ af_status execute(af_plan *p, const void *d_in, void *d_out, uint8_t *d_masks, h
     AF_HIP(launch_seg_reduce(d.dtype, p->d_part, p->d_seg_items, p->nseg, p->Csub, p->d_segsum, st));
   }
   AF_TRY(record(p, set, 2, st));
-  if (iq) {
-    if (p->nlags > 0) {
-      AF_HIP(launch_iqrm_lags(d.dtype, p->d_segsum, p->d_segs, p->d_wseg, p->d_wt0, p->d_wt1,
-                              p->nwin, p->Csub, d.iqrm_mode, p->d_lags, p->nlags,
-                              std::max(0.0f, d.iqrm_nsigma), p->d_flags, p->iqrm_lds, st));
-    }
-    AF_HIP(launch_iqrm_merge(p->d_flags, p->d_lags, p->nlags, p->nwin, p->Csub, d.iqrm_c0,
-                             d.nchan, p->d_masks, st));
-    AF_HIP(launch_segmask(p->d_masks, p->d_segs, p->nseg, d.nchan, p->d_segmask, st));
-  }
+  if (iq) AF_TRY(run_iqrm_vote(p, st));
   AF_TRY(record(p, set, 3, st));
   CleanArgs ca{};
   ca.x = src;
@@ -527,9 +518,30 @@ af_status execute(af_plan *p, const void *d_in, void *d_out, uint8_t *d_masks, h
   ca.dd_c1 = d.dd_c1;
   AF_HIP(launch_clean(d.dtype, ca, st));
   AF_TRY(record(p, set, 4, st));
+  AF_TRY(run_dedisp(p, p->d_xT, p->Lp, d_out, st, set));
+  if (d_masks && iq)
+    AF_HIP(hipMemcpyAsync(d_masks, p->d_masks, (size_t)p->nwin * d.nchan, hipMemcpyDeviceToDevice, st));
+  return AF_OK;
+}
+
+af_status run_iqrm_vote(af_plan *p, hipStream_t st) {
+  const af_plan_desc &d = p->d;
+  if (p->nlags > 0) {
+    AF_HIP(launch_iqrm_lags(d.dtype, p->d_segsum, p->d_segs, p->d_wseg, p->d_wt0, p->d_wt1,
+                            p->nwin, p->Csub, d.iqrm_mode, p->d_lags, p->nlags,
+                            std::max(0.0f, d.iqrm_nsigma), p->d_flags, p->iqrm_lds, st));
+  }
+  AF_HIP(launch_iqrm_merge(p->d_flags, p->d_lags, p->nlags, p->nwin, p->Csub, d.iqrm_c0,
+                           d.nchan, p->d_masks, st));
+  AF_HIP(launch_segmask(p->d_masks, p->d_segs, p->nseg, d.nchan, p->d_segmask, st));
+  return AF_OK;
+}
+
+af_status run_dedisp(af_plan *p, const void *xT, size_t Lp, void *d_out, hipStream_t st, int set) {
+  const af_plan_desc &d = p->d;
   DdArgs da{};
-  da.xT = p->d_xT;
-  da.Lp = p->Lp;
+  da.xT = xT;
+  da.Lp = Lp;
   da.delays = p->d_delays;
   da.nch = p->nch;
   da.ntiles = p->ntiles1;
@@ -577,8 +589,6 @@ af_status execute(af_plan *p, const void *d_in, void *d_out, uint8_t *d_masks, h
     AF_HIP(launch_stage2(d.dtype, sa, st));
   }
   AF_TRY(record(p, set, 6, st));
-  if (d_masks && iq)
-    AF_HIP(hipMemcpyAsync(d_masks, p->d_masks, (size_t)p->nwin * d.nchan, hipMemcpyDeviceToDevice, st));
   return AF_OK;
 }
 

@@ -40,7 +40,32 @@ struct Beam {
   uint64_t total = 0;       // rows uploaded so far (dispatch order)
   uint64_t next_j = 0;      // next output chunk (window start next_j * C)
   size_t plane_bytes = 0, mask_bytes = 0;
-  size_t resident = 0;      // buffer + plan scratch
+  size_t resident = 0;      // buffers + plan scratch
+
+  // Incremental RFI state (u8, ds 1, chunk a multiple of 256 rows, IQRM off
+  // or one window over the whole buffer).  Results are identical to the
+  // plan's own execute on the window:
+  //  - IQRM sums are exact integers, kept per chunk (first r = W mod C rows,
+  //    and the rest) in a ring; a window's sums add whole chunks plus the
+  //    head of the chunk that completes it;
+  //  - the cleaned channel-major rows of window j-1 that overlap window j
+  //    were cleaned with window j-1's mask; when window j's mask is the same
+  //    (decided on the device) only the new rows are cleaned.
+  bool fast = false;
+  uint8_t *d_xT = nullptr;  // [nch][Lcap], window j at column xbase
+  size_t Lcap = 0, xbase = 0;
+  size_t r = 0;             // W mod C
+  int R = 0;                // ring slots
+  Part *d_ring = nullptr;   // [R][2][Csub]
+  uint2 *d_cpart = nullptr; // per-item partials of one chunk
+  RowItem *d_citems = nullptr;
+  int *d_cseg = nullptr;    // {0, head items, all items}
+  int ncitems = 0;
+  RowItem *d_items_old = nullptr, *d_items_new = nullptr;
+  int n_old = 0, n_new = 0;
+  uint8_t *d_prev = nullptr;
+  int *d_flag = nullptr;    // {valid, changed}
+  std::vector<void *> allocs;
 };
 
 struct Req {
@@ -68,6 +93,7 @@ struct af_server {
   int device = 0;
   size_t budget = 0, used = 0, peak = 0;
   int max_inflight = 16, max_queue = 4096;
+  bool incremental = true;
 
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -123,6 +149,61 @@ void *plane_get(af_server *s, size_t bytes) {  // under lock
   return p;
 }
 
+// Window j of a fast beam, completed by chunk k (already uploaded).
+af_status fast_window(af_server *s, Beam &b, Flight &f, uint64_t k) {
+  (void)s;
+  af_plan *p = b.plan;
+  const af_plan_desc &d = p->d;
+  f.j = b.next_j++;
+  const uint64_t j = f.j;
+  const size_t start = (size_t)(j * b.C - b.head);
+  const uint8_t *xw = b.d_buf + start * b.row_bytes;
+  if (d.use_iqrm) {
+    const int nfull = (int)(k - j);
+    AF_HIP(launch_ring_sum(b.d_ring, b.R, (int)(j % b.R), nfull, (int)(k % b.R), b.r == 0 ? 1 : 0,
+                           p->Csub, p->d_segsum, b.st));
+    AF_TRY(run_iqrm_vote(p, b.st));
+  }
+  AF_HIP(launch_mask_delta(p->d_masks, b.d_prev, d.use_iqrm ? d.nchan : 0, b.d_flag, b.st));
+  // window j starts C columns after window j-1; move the overlap back to
+  // column 0 when the next window would not fit (disjoint: xbase > Lp + C)
+  if (j > 0) b.xbase += b.C;
+  if (b.xbase + p->Lp > b.Lcap) {
+    // the copy rounds each row up to 16 bytes: the extra bytes land in rows
+    // [W - C, W), which the new-row clean below rewrites
+    AF_HIP(launch_copy_rows(b.d_xT, b.d_xT + b.xbase, b.Lcap, b.W - b.C, p->nch, b.st));
+    b.xbase = 0;
+  }
+  CleanArgs ca{};
+  ca.x = xw;
+  ca.xT = b.d_xT + b.xbase;
+  ca.Lp = b.Lcap;
+  ca.nchan = d.nchan;
+  ca.segmask = d.use_iqrm ? p->d_segmask : nullptr;
+  ca.stat_mask = p->d_static;
+  ca.zdm = d.use_zero_dm && d.zdm_c1 > d.zdm_c0;
+  ca.zc0 = d.zdm_c0;
+  ca.zc1 = d.zdm_c1;
+  ca.dd_c0 = d.dd_c0;
+  ca.dd_c1 = d.dd_c1;
+  if (b.n_old) {
+    ca.items = b.d_items_old;
+    ca.nitems = b.n_old;
+    ca.run_if = b.d_flag + 1;   // only when the mask changed
+    AF_HIP(launch_clean_u8(ca, b.st));
+  }
+  ca.items = b.d_items_new;
+  ca.nitems = b.n_new;
+  ca.run_if = nullptr;
+  AF_HIP(launch_clean_u8(ca, b.st));
+  AF_HIP(launch_zero_rows(b.d_xT + b.xbase + b.W, b.Lcap, p->Lp - b.W, p->nch, b.st));
+  AF_TRY(run_dedisp(p, b.d_xT + b.xbase, b.Lcap, f.plane, b.st, -1));
+  if (f.masks && d.use_iqrm)
+    AF_HIP(hipMemcpyAsync(f.masks, p->d_masks, (size_t)p->nwin * d.nchan, hipMemcpyDeviceToDevice,
+                          b.st));
+  return AF_OK;
+}
+
 af_status dispatch(af_server *s, Flight &f) {
   Beam &b = s->beams[f.r.beam];
   const size_t in_buf = (size_t)(b.total - b.head);
@@ -136,10 +217,23 @@ af_status dispatch(af_server *s, Flight &f) {
                             hipMemcpyDeviceToDevice, b.st));
     b.head = keep;
   }
-  AF_HIP(hipMemcpyAsync(b.d_buf + (size_t)(b.total - b.head) * b.row_bytes, f.r.h,
-                        b.C * b.row_bytes, hipMemcpyHostToDevice, b.st));
+  const size_t crow = (size_t)(b.total - b.head);   // buffer row of this chunk
+  // host rows (page-locked: asynchronous) or rows already in device memory
+  AF_HIP(hipMemcpyAsync(b.d_buf + crow * b.row_bytes, f.r.h, b.C * b.row_bytes, hipMemcpyDefault,
+                        b.st));
+  const uint64_t k = b.total / b.C;                  // beam chunk index
   b.total += b.C;
-  if (f.plane) {
+  if (b.fast) {
+    af_plan *p = b.plan;
+    if (p->d.use_iqrm) {
+      // this chunk's sums: head rows [0, r) and the rest
+      AF_HIP(launch_seg_stats_u8(b.d_buf + crow * b.row_bytes, p->d.nchan, p->d.iqrm_c0,
+                                 p->d.iqrm_c1, b.d_citems, b.ncitems, b.d_cpart, b.st));
+      AF_HIP(launch_seg_reduce_u8(b.d_cpart, b.d_cseg, 2, p->Csub,
+                                  b.d_ring + (size_t)(k % b.R) * 2 * p->Csub, b.st));
+    }
+    if (f.plane) AF_TRY(fast_window(s, b, f, k));
+  } else if (f.plane) {
     f.j = b.next_j++;
     const size_t start = (size_t)(f.j * b.C - b.head);
     AF_TRY(execute(b.plan, b.d_buf + start * b.row_bytes, f.plane, f.masks, b.st));
@@ -271,6 +365,7 @@ af_status af_server_create(const af_server_desc *desc, af_server **srv) {
   (void)hipGetDevice(&s->device);
   if (desc && desc->max_inflight > 0) s->max_inflight = desc->max_inflight;
   if (desc && desc->max_queue > 0) s->max_queue = desc->max_queue;
+  if (desc && desc->full_rfi) s->incremental = false;
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
     delete s;
@@ -281,6 +376,89 @@ af_status af_server_create(const af_server_desc *desc, af_server **srv) {
   *srv = s;
   return AF_OK;
 }
+
+extern "C++" {  // helpers of af_server_add_beam (C++ linkage inside the C-ABI block)
+
+static void release_beam(af::Beam &b) {
+  if (b.st) (void)hipStreamSynchronize(b.st);
+  if (b.plan) af_plan_destroy(b.plan);
+  if (b.d_buf) (void)hipFree(b.d_buf);
+  for (void *q : b.allocs) (void)hipFree(q);
+  if (b.st) (void)hipStreamDestroy(b.st);
+  b = af::Beam{};
+}
+
+// Rows [r0, r1) cut into items of at most `step` rows (segment 0).
+static std::vector<af::RowItem> row_items(size_t r0, size_t r1, size_t step) {
+  std::vector<af::RowItem> v;
+  for (size_t r = r0; r < r1; r += step)
+    v.push_back(af::RowItem{0u, (uint32_t)r, (uint32_t)std::min(r1, r + step), 0u});
+  return v;
+}
+
+template <typename T>
+static af_status beam_alloc(af::Beam &b, T **ptr, size_t bytes) {
+  void *q = nullptr;
+  if (hipMalloc(&q, bytes ? bytes : 1) != hipSuccess) return fail(AF_ERR_NOMEM, "hipMalloc beam state");
+  b.allocs.push_back(q);
+  *ptr = (T *)q;
+  return AF_OK;
+}
+
+template <typename T>
+static af_status beam_upload(af::Beam &b, T **ptr, const std::vector<T> &v) {
+  AF_TRY(beam_alloc(b, ptr, v.size() * sizeof(T)));
+  if (!v.empty()) AF_HIP(hipMemcpy(*ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return AF_OK;
+}
+
+// Incremental RFI state for beams that qualify; returns its device bytes
+// through `bytes` (0 when the beam runs the plan's full execute per window).
+static af_status setup_fast(af::Beam &b, size_t *bytes) {
+  *bytes = 0;
+  af_plan *p = b.plan;
+  const af_plan_desc &d = p->d;
+  const bool ok = d.dtype == AF_U8 && d.ds == 1 && b.C % 256 == 0 &&
+                  d.nchan <= af::kCleanU8MaxChan && p->nch == d.dd_c1 - d.dd_c0 &&
+                  (!d.use_iqrm || (p->nwin == 1 && p->nseg == 1 && p->Csub > 0));
+  if (!ok) return AF_OK;
+  b.Lcap = ((2 * p->Lp + b.C) + 255) & ~(size_t)255;
+  b.r = b.W % b.C;
+  b.R = (int)((b.W + b.C - 1) / b.C) + 2;
+  AF_TRY(beam_alloc(b, &b.d_xT, (size_t)p->nch * b.Lcap));
+  *bytes += (size_t)p->nch * b.Lcap;
+  if (d.use_iqrm) {
+    // 64-row items: a chunk is a small grid, so more, shorter items
+    std::vector<af::RowItem> it = row_items(0, b.r, af::kStatRowsU8 / 4);
+    const int nhead = (int)it.size();
+    std::vector<af::RowItem> rest = row_items(b.r, b.C, af::kStatRowsU8 / 4);
+    it.insert(it.end(), rest.begin(), rest.end());
+    b.ncitems = (int)it.size();
+    AF_TRY(beam_upload(b, &b.d_citems, it));
+    AF_TRY(beam_upload(b, &b.d_cseg, std::vector<int>{0, nhead, b.ncitems}));
+    AF_TRY(beam_alloc(b, &b.d_cpart, (size_t)b.ncitems * p->Csub * sizeof(uint2)));
+    AF_TRY(beam_alloc(b, &b.d_ring, (size_t)b.R * 2 * p->Csub * sizeof(af::Part)));
+    *bytes += (size_t)b.ncitems * p->Csub * sizeof(uint2) + (size_t)b.R * 2 * p->Csub * sizeof(af::Part);
+  }
+  // the new rows' items start on a 32-row boundary at or before W - C; the
+  // few overlap rows they repeat come out identical
+  const size_t A = ((b.W - b.C) / af::kCleanRowsU8) * af::kCleanRowsU8;
+  std::vector<af::RowItem> old_items = row_items(0, A, af::kCleanRowsU8);
+  std::vector<af::RowItem> new_items = row_items(A, b.W, af::kCleanRowsU8);
+  b.n_old = (int)old_items.size();
+  b.n_new = (int)new_items.size();
+  AF_TRY(beam_upload(b, &b.d_items_old, old_items));
+  AF_TRY(beam_upload(b, &b.d_items_new, new_items));
+  AF_TRY(beam_alloc(b, &b.d_prev, (size_t)d.nchan));
+  AF_TRY(beam_alloc(b, &b.d_flag, 2 * sizeof(int)));
+  AF_HIP(hipMemset(b.d_prev, 0, (size_t)d.nchan));
+  AF_HIP(hipMemset(b.d_flag, 0, 2 * sizeof(int)));
+  *bytes += (old_items.size() + new_items.size()) * sizeof(af::RowItem) + d.nchan + 8;
+  b.fast = true;
+  return AF_OK;
+}
+
+}  // extern "C++"
 
 af_status af_server_add_beam(af_server *s, const af_plan_desc *w, size_t chunk_rows, int *beam) {
   if (!s || !w || !beam) return fail(AF_ERR_INVALID, "null argument");
@@ -305,20 +483,28 @@ af_status af_server_add_beam(af_server *s, const af_plan_desc *w, size_t chunk_r
   b.plane_bytes = ((size_t)info.ndm * info.nout * info.out_elem_bytes + 255) & ~(size_t)255;
   b.mask_bytes = (size_t)info.nwin * w->nchan;
   b.resident = b.rcap * b.row_bytes + info.scratch_bytes;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->used + b.resident > s->budget) {
+      release_beam(b);
+      return fail(AF_ERR_NOMEM, "beam does not fit the HBM budget");
+    }
+  }
+  af_status st = AF_OK;
+  size_t extra = 0;
+  if (hipMalloc(&b.d_buf, b.rcap * b.row_bytes) != hipSuccess)
+    st = fail(AF_ERR_NOMEM, "hipMalloc beam buffer");
+  if (st == AF_OK && hipStreamCreateWithFlags(&b.st, hipStreamNonBlocking) != hipSuccess)
+    st = fail(AF_ERR_RUNTIME, "hipStreamCreate");
+  if (st == AF_OK && s->incremental) st = setup_fast(b, &extra);
   std::lock_guard<std::mutex> lk(s->mu);
-  if (s->used + b.resident > s->budget) {
-    af_plan_destroy(b.plan);
-    return fail(AF_ERR_NOMEM, "beam does not fit the HBM budget");
+  if (st == AF_OK && s->used + b.resident + extra > s->budget)
+    st = fail(AF_ERR_NOMEM, "beam does not fit the HBM budget");
+  if (st != AF_OK) {
+    release_beam(b);
+    return st;
   }
-  if (hipMalloc(&b.d_buf, b.rcap * b.row_bytes) != hipSuccess) {
-    af_plan_destroy(b.plan);
-    return fail(AF_ERR_NOMEM, "hipMalloc beam buffer");
-  }
-  if (hipStreamCreateWithFlags(&b.st, hipStreamNonBlocking) != hipSuccess) {
-    (void)hipFree(b.d_buf);
-    af_plan_destroy(b.plan);
-    return fail(AF_ERR_RUNTIME, "hipStreamCreate");
-  }
+  b.resident += extra;
   s->used += b.resident;
   s->peak = std::max(s->peak, s->used);
   s->beams.push_back(b);
@@ -420,12 +606,7 @@ af_status af_server_destroy(af_server *s) {
   }
   if (s->thr.joinable()) s->thr.join();
   (void)hipSetDevice(s->device);
-  for (auto &b : s->beams) {
-    if (b.st) (void)hipStreamSynchronize(b.st);
-    if (b.plan) af_plan_destroy(b.plan);
-    if (b.d_buf) (void)hipFree(b.d_buf);
-    if (b.st) (void)hipStreamDestroy(b.st);
-  }
+  for (auto &b : s->beams) release_beam(b);
   for (auto &kv : s->plane_size) (void)hipFree(kv.first);
   for (auto e : s->ev_free) (void)hipEventDestroy(e);
   delete s;

@@ -49,13 +49,16 @@ def make_beams(nbeams: int = 19, nchan: int = 4096, dm_max: float = 5000.0,
                dm_step: float = 1.0, durations=(0.25, 0.5, 1.0), seed: int = 5,
                sbw: int = 128) -> List[StreamBeam]:
     """19 beams x 4096 channels over 1000-1500 MHz, tsamp 49.152 us; each
-    beam's chunk duration drawn from `durations` (seeded)."""
+    beam's chunk duration drawn from `durations` (seeded) and rounded up to
+    whole 256-sample blocks (0.25/0.5/1.0 s -> 5120/10240/20480 samples),
+    the chunk shape the server's incremental RFI path takes."""
     band = synth.band_1000_1500(nchan)
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(nbeams):
         dur = float(rng.choice(durations))
-        c = int(math.ceil(dur / float(band.tsamp)))
+        c = int(math.ceil(dur / float(band.tsamp) / 256)) * 256
+        dur = c * float(band.tsamp)
         spec = window_spec(band, c, dm_max, dm_step, sbw=sbw)
         out.append(StreamBeam(spec, c, dur, spec.nsamp))
     return out
@@ -66,22 +69,35 @@ class RowPool:
     offset into it.  10% of chunks (seeded) carry a dispersed pulse in their
     own private copy of the rows."""
 
-    def __init__(self, nchan: int, rows: int, seed: int = 5, threads: int = 16):
+    def __init__(self, nchan: int, rows: int, seed: int = 5, threads: int = 16,
+                 device: bool = False):
         self.nchan = nchan
         self.rows = rows
+        self.device = device
         self.buf = capi.HostBuffer((rows, nchan), np.uint8)
         self.buf.array[:] = synth.noise_u8_threaded(seed, rows, nchan, threads=threads)
+        self.dbuf = None
+        if device:   # rows already in HBM (a capture path writing to the device)
+            self.dbuf = capi.DeviceBuffer(rows * nchan)
+            self.dbuf.upload(self.buf.array)
         self.extra = []
 
     def chunk_ptr(self, rng: np.random.Generator, nrows: int, pulse_dm: float = 0.0,
                   band: synth.Band = None) -> int:
         off = int(rng.integers(0, self.rows - nrows + 1))
+        base = self.dbuf.ptr if self.device else self.buf.ptr
         if pulse_dm <= 0:
-            return self.buf.ptr + off * self.nchan
+            return base + off * self.nchan
         hb = capi.HostBuffer((nrows, self.nchan), np.uint8)
         hb.array[:] = self.buf.array[off: off + nrows]
         synth.inject_pulse(hb.array, band.freq, band.f_top, float(band.tsamp), pulse_dm,
                            nrows // 2, 1e-3, 48.0)
+        if self.device:
+            db = capi.DeviceBuffer(hb.array.nbytes)
+            db.upload(hb.array)
+            hb.free()
+            self.extra.append(db)
+            return db.ptr
         self.extra.append(hb)
         return hb.ptr
 
@@ -89,6 +105,8 @@ class RowPool:
         for hb in self.extra:
             hb.free()
         self.extra = []
+        if self.dbuf is not None:
+            self.dbuf.free()
         self.buf.free()
 
 
@@ -121,6 +139,12 @@ def run_rate(srv: capi.Server, beams: List[StreamBeam], pool: RowPool, rate: flo
     rng.shuffle(order)
     gaps = rng.exponential(1.0 / agg, n_total) if rate > 0 else np.zeros(n_total)
     t_arr = np.cumsum(gaps)
+    # chunk rows are prepared before the clock starts (pulse chunks need
+    # their own buffers)
+    ptrs = []
+    for b in order:
+        pulse = float(rng.uniform(50, 5000)) if rng.random() < 0.1 else 0.0
+        ptrs.append(pool.chunk_ptr(rng, beams[b].chunk_rows, pulse, band))
     base = srv.stats()
     lat, done, work = [], 0, 0.0
     t0 = time.perf_counter()
@@ -136,8 +160,7 @@ def run_rate(srv: capi.Server, beams: List[StreamBeam], pool: RowPool, rate: flo
                 break
             time.sleep(min(dt, 0.0005))
         sb = beams[b]
-        pulse = float(rng.uniform(50, 5000)) if rng.random() < 0.1 else 0.0
-        srv.submit(int(b), pool.chunk_ptr(rng, sb.chunk_rows, pulse, band), tag=i)
+        srv.submit(int(b), ptrs[i], tag=i)
         work += sb.spec.ndm * sb.chunk_rows
     while done < n_total:
         for r in srv.poll(64, 1000):

@@ -188,6 +188,7 @@ typedef struct af_server_desc {
   size_t hbm_budget;       /* bytes for beam buffers + result planes; 0 = 90% of free memory */
   int max_inflight;        /* chunks dispatched and not finished (0 = 16)              */
   int max_queue;           /* submitted chunks waiting for admission (0 = 4096)        */
+  int full_rfi;            /* 1: every window runs the whole plan (no incremental RFI) */
 } af_server_desc;
 
 /* One finished chunk.  Chunks submitted before a beam's first window is full
@@ -225,8 +226,10 @@ af_status af_server_add_beam(af_server *srv, const af_plan_desc *window, size_t 
                              int *beam);
 /* Queues the beam's next chunk_rows rows ([chunk_rows][nchan], time-major).
  * The rows are borrowed, not copied: keep them unchanged until this chunk's
- * result comes back from af_server_poll (page-locked memory from
- * af_host_alloc makes the upload asynchronous).  AF_ERR_BUSY: queue full. */
+ * result comes back from af_server_poll.  They may be host memory
+ * (page-locked memory from af_host_alloc makes the upload asynchronous) or
+ * device memory (rows written there by a capture path: a device copy).
+ * AF_ERR_BUSY: queue full. */
 af_status af_server_submit(af_server *srv, int beam, const void *h_rows, uint64_t tag);
 /* Returns up to max finished chunks, in completion order; waits up to
  * wait_ms for the first one. */

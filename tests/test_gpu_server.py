@@ -24,18 +24,24 @@ def beam_stream(seed, nchan, nrows, band):
     x = synth.noise_u8(rng, nrows, nchan)
     synth.inject_narrowband_rfi(x, rng, frac=0.05, amp=40, n_stuck=2, pmax=256)
     synth.inject_pulse(x, band.freq, band.f_top, float(band.tsamp), 60.0, nrows // 2, 2e-3, 40.0)
+    # a channel that turns noisy half way: windows after it flag it, so the
+    # served masks change from window to window at least once
+    t = np.arange(nrows // 2, nrows)
+    x[nrows // 2:, nchan // 3 + 1] = np.clip(128 + 90 * np.sin(t * 0.7), 0, 255).astype(np.uint8)
     return x
 
 
-# (seed, nchan, chunk rows, DM step, ndm, method, extra spec fields)
+# (seed, nchan, chunk rows, DM step, ndm, method, extra spec fields).  Chunks
+# of a multiple of 256 rows take the server's incremental RFI path (beams 0
+# and 2); beam 1 runs the plan's full execute on every window.
 BEAMS = [
-    (11, 256, 600, 0.5, 161, "subband",
+    (11, 256, 512, 0.5, 161, "subband",
      dict(sbw=32, ndm0=16, use_iqrm=True, iqrm_c0=0, iqrm_c1=255, iqrm_mode=1,
           iqrm_radius_frac=np.float32(5.5 / 255), iqrm_nsigma=3.0,
           use_zero_dm=True, zdm_c0=0, zdm_c1=255)),
     (12, 128, 1000, 1.0, 90, "direct",
      dict(use_zero_dm=True, zdm_c0=0, zdm_c1=128, bad_channels=[3, 77])),
-    (13, 512, 350, 0.25, 240, "subband",
+    (13, 512, 256, 0.25, 240, "subband",
      dict(sbw=64, ndm0=32, use_iqrm=True, iqrm_c0=0, iqrm_c1=511, iqrm_mode=0,
           iqrm_radius_frac=np.float32(0.02), iqrm_nsigma=3.0)),
 ]
@@ -108,10 +114,12 @@ def _collect(srv, beams, r, out):
     out[(r.beam, r.window)] = (plane, masks)
 
 
-def test_served_results_equal_plain_calls(gpu_lib):
-    say("case: server, 3 beams interleaved, every window vs the plain call and the oracle")
-    beams = make_beams(nchunks=5)
-    srv = capi.Server()
+@pytest.mark.parametrize("full_rfi", [False, True])
+def test_served_results_equal_plain_calls(gpu_lib, full_rfi):
+    say(f"case: server (full_rfi={full_rfi}), 3 beams interleaved, every window vs the plain "
+        "call and the oracle")
+    beams = make_beams(nchunks=6)
+    srv = capi.Server(full_rfi=full_rfi)
     host = []
     for b in beams:
         srv.add_beam(b["spec"], b["C"])
@@ -141,6 +149,12 @@ def test_served_results_equal_plain_calls(gpu_lib):
                     assert np.array_equal(masks, rmask), (i, j)
         plan.close()
         assert sum(1 for (bi, _) in got if bi == i) == nwin
+        if i == 0:
+            # both incremental branches ran: reused overlap (same mask as the
+            # previous window) and a full re-clean (mask changed); beam 2's
+            # mean-mode masks change at every window
+            same = [np.array_equal(got[(i, j)][1], got[(i, j - 1)][1]) for j in range(1, nwin)]
+            assert any(same) and not all(same), (i, same)
     assert st["planes"] == len(got) and st["p99_ms"] >= st["p50_ms"] > 0
     for hb in host:
         hb.free()
