@@ -66,8 +66,8 @@ def build_core(verbose=False):
     import pybind11  # noqa: WPS433
     out_path = core_module_path()
     tmp = out_path + ".tmp"
-    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
-           f"-I{INC}", f"-I{pybind11.get_include()}",
+    cmd = ["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fopenmp",
+           "-fvisibility=hidden", f"-I{INC}", f"-I{CSRC}", f"-I{pybind11.get_include()}",
            f"-I{sysconfig.get_paths()['include']}",
            src, "-o", tmp, f"-L{PKG}", "-lastroflow_amd", "-Wl,-rpath,$ORIGIN"]
     out = _run(cmd)

@@ -1,0 +1,564 @@
+// af_bindings.cpp -- the `_astroflow_core` Python module on top of the C-ABI.
+//
+// Mirrors the reference's pybind11 surface for the dedispersion path
+// (src/bindpy/bindings.cpp:13-74, src/bindpy/pyapi.cpp:53-282): the same
+// module name, class names, constructors and positional argument order, so
+// python/astroflow's callers (dedispered.py:59-72,360-373) run unchanged.
+// Every call goes through libastroflow_amd (HIP kernels for gfx950); there is
+// no CPU compute path.  Additions: `_dedisperse_spec_raw` returns the plane
+// itself (and the IQRM masks), which the reference never exposes.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "af_image.h"
+#include "astroflow_amd.h"
+
+namespace py = pybind11;
+
+namespace {
+
+// ------------------------------------------------------------ config types
+
+// include/data.h:24-50
+struct Header {
+  float mjd = 0, tsamp = 0, fch1 = 0, foff = 0;
+  int nbits = 0, nchans = 0;
+  long ndata = 0;
+  std::string filename;
+  Header() = default;
+  Header(float mjd_, float tsamp_, float fch1_, float foff_, int nbits_, int nchans_, long ndata_,
+         std::string fn)
+      : mjd(mjd_), tsamp(tsamp_), fch1(fch1_), foff(foff_), nbits(nbits_), nchans(nchans_),
+        ndata(ndata_), filename(std::move(fn)) {}
+  explicit Header(py::object h)
+      : mjd(h.attr("mjd").cast<float>()), tsamp(h.attr("tsamp").cast<float>()),
+        fch1(h.attr("fch1").cast<float>()), foff(h.attr("foff").cast<float>()),
+        nbits(h.attr("nbits").cast<int>()), nchans(h.attr("nchans").cast<int>()),
+        ndata(h.attr("ndata").cast<long>()), filename(h.attr("filename").cast<std::string>()) {}
+};
+
+// include/rfi.h:12-57 (same defaults and positional order)
+struct IQRMConfig {
+  int mode = 0;
+  float radius_frac = 0.10f, nsigma = 3.0f;
+  double geofactor = 1.5, win_sec = 5.0, hop_sec = 2.5;
+  bool include_tail = true;
+  IQRMConfig() = default;
+  IQRMConfig(int m, float r, float n, double g, double w, double h, bool t)
+      : mode(m), radius_frac(r), nsigma(n), geofactor(g), win_sec(w), hop_sec(h), include_tail(t) {}
+  explicit IQRMConfig(py::object c)
+      : mode(c.attr("mode").cast<int>()), radius_frac(c.attr("radius_frac").cast<float>()),
+        nsigma(c.attr("nsigma").cast<float>()), geofactor(c.attr("geofactor").cast<double>()),
+        win_sec(c.attr("win_sec").cast<double>()), hop_sec(c.attr("hop_sec").cast<double>()),
+        include_tail(c.attr("include_tail").cast<bool>()) {}
+};
+
+struct RFIConfig {
+  bool use_mask = false, use_zero_dm = false, use_iqrm = false;
+  IQRMConfig iqrm_cfg;
+  RFIConfig() = default;
+  RFIConfig(bool m, bool z, bool i, IQRMConfig c)
+      : use_mask(m), use_zero_dm(z), use_iqrm(i), iqrm_cfg(c) {}
+  explicit RFIConfig(py::object c)
+      : use_mask(c.attr("use_mask").cast<bool>()), use_zero_dm(c.attr("use_zero_dm").cast<bool>()),
+        use_iqrm(c.attr("use_iqrm").cast<bool>()), iqrm_cfg(IQRMConfig(c.attr("iqrm_cfg"))) {}
+};
+
+// include/data.h:52-83
+struct DedispersedDataUint8 {
+  py::list dm_times;               // (512, 512, 3) uint8 BGR images
+  std::vector<size_t> shape;
+  int dm_ndata = 0, downtsample_ndata = 0;
+  float dm_low = 0, dm_high = 0, dm_step = 0, tsample = 0;
+  std::string filname;
+};
+
+struct DedisperedData {
+  py::list dm_times;               // one flat plane [dm * time]
+  std::vector<size_t> shape;
+  int dm_ndata = 0, downtsample_ndata = 0;
+  float dm_low = 0, dm_high = 0, dm_step = 0, tsample = 0;
+  std::string filname;
+};
+
+// ------------------------------------------------------------------ errors
+
+[[noreturn]] void raise_status(af_status s, const char *what) {
+  std::string msg = std::string(what) + ": " + af_last_error();
+  if (s == AF_ERR_INVALID) throw py::value_error(msg);
+  if (s == AF_ERR_NOMEM) throw std::bad_alloc();
+  throw std::runtime_error(msg);
+}
+
+void check(af_status s, const char *what) {
+  if (s != AF_OK) raise_status(s, what);
+}
+
+template <typename T> constexpr int dtype_code();
+template <> constexpr int dtype_code<uint8_t>() { return AF_U8; }
+template <> constexpr int dtype_code<uint16_t>() { return AF_U16; }
+template <> constexpr int dtype_code<uint32_t>() { return AF_U32; }
+template <> constexpr int dtype_code<float>() { return AF_F32; }
+
+template <typename T> struct OutOf { using type = uint32_t; };
+template <> struct OutOf<uint32_t> { using type = uint64_t; };
+template <> struct OutOf<float> { using type = float; };
+
+// Whitespace-separated channel numbers (src/rfi/rfimarker.cu:441-466); a
+// missing or empty file means no bad channels.
+std::vector<int> read_mask_file(const std::string &path) {
+  std::vector<int> v;
+  std::ifstream f(path);
+  if (!f.is_open()) return v;
+  std::string line;
+  while (std::getline(f, line)) {
+    std::stringstream ss(line);
+    int c;
+    while (ss >> c) v.push_back(c);
+  }
+  return v;
+}
+
+// ------------------------------------------------------- the plain call --
+
+struct Prepared {
+  af_plan_desc d{};
+  std::vector<double> freq;
+  std::vector<int> bad;
+  size_t chan_start = 0, chan_end = 0, dm_steps = 0, down_ndata = 0;
+};
+
+// Channel range, DM steps, reference frequency and RFI set-up exactly as
+// src/core/dedispered.cu:1003-1036,1085-1118 derive them from the call.
+Prepared prepare(int dtype, const Header &h, size_t nsamp, float dm_low, float dm_high,
+                 float freq_start, float freq_end, float dm_step, int ds,
+                 const std::string &mask_file, const RFIConfig &rfi) {
+  Prepared p;
+  if (h.nchans <= 0) throw py::value_error("header.nchans must be positive");
+  if (ds < 1) throw py::value_error("time_downsample must be >= 1");
+  if (!(dm_step > 0.0f)) throw py::value_error("dm_step must be positive");
+  if (dm_high < dm_low) throw py::value_error("dm_high must be >= dm_low");
+  const size_t nchans = (size_t)h.nchans;
+  p.freq.resize(nchans);
+  for (size_t i = 0; i < nchans; ++i) {
+    const float f = h.fch1 + (float)i * h.foff;   // float Header fields
+    p.freq[i] = f;
+  }
+  const float fmin = (float)p.freq.front(), fmax = (float)p.freq.back();
+  if (freq_start < fmin || freq_end > fmax) {
+    char msg[256];
+    std::snprintf(msg, sizeof(msg),
+                  "Frequency range [%.3f-%.3f MHz] out of spectrum range [%.3f-%.3f MHz]",
+                  freq_start, freq_end, fmin, fmax);
+    throw py::value_error(msg);
+  }
+  const float span = fmax - fmin;
+  size_t cs = (size_t)((freq_start - fmin) / span * (float)(nchans - 1));
+  size_t ce = (size_t)((freq_end - fmin) / span * (float)(nchans - 1));
+  ce = std::min(nchans - 1, ce);
+  if (cs >= nchans || ce >= nchans || cs > ce) {
+    char msg[256];
+    std::snprintf(msg, sizeof(msg), "Invalid channel range [%zu-%zu] for %zu channels", cs, ce,
+                  nchans);
+    throw py::value_error(msg);
+  }
+  p.chan_start = cs;
+  p.chan_end = ce;
+  p.dm_steps = (size_t)((dm_high - dm_low) / dm_step) + 1;
+  p.down_ndata = (nsamp + (size_t)ds - 1) / (size_t)ds;
+  af_plan_desc &d = p.d;
+  d.dtype = dtype;
+  d.nchan = h.nchans;
+  d.nsamp = nsamp;
+  d.ds = ds;
+  d.tsamp = h.tsamp;
+  d.f_ref_mhz = p.freq[ce];
+  d.dd_c0 = (int)cs;
+  d.dd_c1 = (int)ce + 1;
+  d.dm_low = dm_low;
+  d.dm_step = dm_step;
+  d.ndm = (int)p.dm_steps;
+  d.nout = 0;
+  d.method = AF_METHOD_SUBBAND;
+  d.sb_nchan = 32;   // src/core/dedispered.cu:29-35
+  d.ndm0 = 32;
+  d.use_iqrm = rfi.use_iqrm;
+  d.iqrm_c0 = (int)cs;   // IQRM takes the inclusive end as exclusive (:1089)
+  d.iqrm_c1 = (int)ce;
+  d.iqrm_mode = rfi.iqrm_cfg.mode;
+  d.iqrm_radius_frac = rfi.iqrm_cfg.radius_frac;
+  d.iqrm_nsigma = rfi.iqrm_cfg.nsigma;
+  d.iqrm_geofactor = rfi.iqrm_cfg.geofactor;
+  // window and hop in binned samples (src/core/iqrm.cu:368-369, with the
+  // binned sample time header.tsamp * time_downsample in float)
+  const double tb = (double)(h.tsamp * (float)ds);
+  const double win = std::max(0.0, rfi.iqrm_cfg.win_sec), hop = std::max(0.0, rfi.iqrm_cfg.hop_sec);
+  const unsigned W = win > 0.0 ? (unsigned)std::floor(win / tb) : 0u;
+  const unsigned H = hop > 0.0 ? (unsigned)std::floor(hop / tb) : W;
+  d.iqrm_win = (W >= p.down_ndata) ? 0u : W;
+  d.iqrm_hop = H;
+  d.iqrm_include_tail = rfi.iqrm_cfg.include_tail;
+  d.use_zero_dm = rfi.use_zero_dm;
+  d.zdm_c0 = (int)cs;
+  d.zdm_c1 = (int)ce;
+  if (rfi.use_mask) p.bad = read_mask_file(mask_file);
+  d.bad_chans = p.bad.empty() ? nullptr : p.bad.data();
+  d.n_bad = (int)p.bad.size();
+  d.freq_mhz = p.freq.data();
+  return p;
+}
+
+struct Raw {
+  py::array plane;
+  py::object masks;
+  Prepared prep;
+};
+
+template <typename T>
+Raw run(py::array_t<T, py::array::c_style> data, const Header &h, float dm_low, float dm_high,
+        float freq_start, float freq_end, float dm_step, int ds, int target_id,
+        const std::string &mask_file, const RFIConfig &rfi, bool want_masks) {
+  const py::buffer_info bi = data.request();
+  if (bi.ndim < 1 || bi.ndim > 3) throw std::runtime_error("data must be 1D, 2D, or 3D");
+  if (h.ndata <= 0) throw py::value_error("header.ndata must be positive");
+  if ((size_t)bi.size < (size_t)h.ndata * (size_t)std::max(h.nchans, 0))
+    throw py::value_error("data holds fewer than header.ndata * header.nchans samples");
+  int ndev = 0;
+  if (af_device_count(&ndev) != AF_OK || ndev == 0) throw std::runtime_error("No HIP devices found");
+  if (target_id < 0 || target_id >= ndev) throw py::value_error("target_id names no device");
+  check(af_set_device(target_id), "af_set_device");
+  Raw r;
+  r.prep = prepare(dtype_code<T>(), h, (size_t)h.ndata, dm_low, dm_high, freq_start, freq_end,
+                   dm_step, ds, mask_file, rfi);
+  af_plan *plan = nullptr;
+  check(af_plan_create(&r.prep.d, &plan), "af_plan_create");
+  std::unique_ptr<af_plan, decltype(&af_plan_destroy)> guard(plan, af_plan_destroy);
+  af_plan_info info;
+  check(af_plan_query(plan, &info), "af_plan_query");
+  using O = typename OutOf<T>::type;
+  py::array_t<O> plane({(py::ssize_t)info.ndm, (py::ssize_t)info.nout});
+  py::object masks = py::none();
+  uint8_t *mp = nullptr;
+  if (want_masks && info.nwin > 0) {
+    py::array_t<uint8_t> m({(py::ssize_t)info.nwin, (py::ssize_t)h.nchans});
+    mp = m.mutable_data();
+    masks = m;
+  }
+  const T *src = static_cast<const T *>(bi.ptr);
+  O *dst = plane.mutable_data();
+  af_status st;
+  {
+    py::gil_scoped_release nogil;
+    st = af_execute_host(plan, src, dst, mp);
+  }
+  check(st, "af_execute_host");
+  r.plane = plane;
+  r.masks = masks;
+  return r;
+}
+
+// Slices of t_sample seconds -> DM-time images (include/data.h:619-727).
+template <typename O>
+py::list images(const py::array_t<O> &plane, float tsamp, int ds, float t_sample) {
+  const size_t rows = (size_t)plane.shape(0), cols = (size_t)plane.shape(1);
+  const float tb = tsamp * (float)ds;
+  const size_t sps = (size_t)(t_sample / tb);
+  if (sps == 0) throw py::value_error("t_sample is shorter than one (downsampled) sample");
+  const size_t nslices = (cols + sps - 1) / sps;
+  constexpr int kSize = 512;
+  std::vector<std::vector<uint8_t>> bufs(nslices, std::vector<uint8_t>((size_t)kSize * kSize * 3));
+  const O *p = plane.data();
+  {
+    py::gil_scoped_release nogil;
+    for (size_t s = 0; s < nslices; ++s) {
+      const size_t c0 = s * sps, c1 = std::min(cols, c0 + sps);
+      afimg::slice_image(p, cols, c0, (int)rows, (int)(c1 - c0), kSize, bufs[s].data());
+    }
+  }
+  py::list out;
+  for (auto &b : bufs) {
+    py::array_t<uint8_t> a({kSize, kSize, 3});
+    std::memcpy(a.mutable_data(), b.data(), b.size());
+    out.append(a);
+  }
+  return out;
+}
+
+template <typename T>
+DedispersedDataUint8 dedisperse_spec(py::array_t<T, py::array::c_style> data, Header header,
+                                     float dm_low, float dm_high, float freq_start, float freq_end,
+                                     float dm_step, int time_downsample, float t_sample,
+                                     int target_id, std::string mask_file, RFIConfig rficfg) {
+  Raw r = run<T>(data, header, dm_low, dm_high, freq_start, freq_end, dm_step, time_downsample,
+                 target_id, mask_file, rficfg, false);
+  using O = typename OutOf<T>::type;
+  DedispersedDataUint8 out;
+  out.dm_times = images<O>(py::array_t<O>(r.plane), header.tsamp, time_downsample, t_sample);
+  out.shape = {512, 512, 3};
+  out.dm_ndata = 512;
+  out.downtsample_ndata = 512;
+  out.dm_low = dm_low;
+  out.dm_high = dm_high;
+  out.dm_step = dm_step;
+  out.tsample = t_sample;
+  out.filname = header.filename;
+  return out;
+}
+
+template <typename T>
+py::object dedisperse_spec_raw(py::array_t<T, py::array::c_style> data, Header header,
+                               float dm_low, float dm_high, float freq_start, float freq_end,
+                               float dm_step, int time_downsample, int target_id,
+                               std::string mask_file, RFIConfig rficfg, bool return_masks) {
+  Raw r = run<T>(data, header, dm_low, dm_high, freq_start, freq_end, dm_step, time_downsample,
+                 target_id, mask_file, rficfg, return_masks);
+  if (return_masks) return py::make_tuple(r.plane, r.masks);
+  return r.plane;
+}
+
+// ---------------------------------------------------- filterbank ingest --
+// SIGPROC filterbank: a keyword header between HEADER_START and HEADER_END
+// (each keyword an int32 length + text, followed by its value), then
+// [ndata][nifs][nchans] samples.  Data with a negative channel offset is
+// flipped to ascending frequency, as src/data/filterbank.cpp:793-842 does.
+struct Filterbank {
+  std::string filename;
+  int nchans = 0, nifs = 1, nbits = 8, telescope_id = 0, machine_id = 0, data_type = 1;
+  double fch1 = 0, foff = 0, tstart = 0, tsamp = 0;
+  long ndata = 0;
+  std::string source_name;
+  std::shared_ptr<std::vector<uint8_t>> raw;   // samples as stored (after the flip)
+
+  Filterbank() = default;
+  explicit Filterbank(const std::string &path) : filename(path) { read(); }
+
+  static bool read_str(std::ifstream &f, std::string &s) {
+    int32_t n = 0;
+    if (!f.read((char *)&n, 4) || n < 0 || n > 4096) return false;
+    s.resize((size_t)n);
+    return (bool)f.read(&s[0], n);
+  }
+
+  void read() {
+    std::ifstream f(filename, std::ios::binary);
+    if (!f.is_open()) throw std::runtime_error("Can not open file " + filename);
+    std::string k;
+    if (!read_str(f, k) || k != "HEADER_START")
+      throw py::value_error("not a SIGPROC filterbank file: " + filename);
+    auto rd_i = [&](int &v) { int32_t x; f.read((char *)&x, 4); v = x; };
+    auto rd_d = [&](double &v) { f.read((char *)&v, 8); };
+    for (;;) {
+      if (!read_str(f, k)) throw py::value_error("truncated filterbank header: " + filename);
+      if (k == "HEADER_END") break;
+      if (k == "nchans") rd_i(nchans);
+      else if (k == "nifs") rd_i(nifs);
+      else if (k == "nbits") rd_i(nbits);
+      else if (k == "telescope_id") rd_i(telescope_id);
+      else if (k == "machine_id") rd_i(machine_id);
+      else if (k == "data_type") rd_i(data_type);
+      else if (k == "fch1") rd_d(fch1);
+      else if (k == "foff") rd_d(foff);
+      else if (k == "tstart") rd_d(tstart);
+      else if (k == "tsamp") rd_d(tsamp);
+      else if (k == "source_name" || k == "rawdatafile") {
+        std::string v;
+        read_str(f, v);
+        if (k == "source_name") source_name = v;
+      } else if (k == "nbeams" || k == "ibeam" || k == "barycentric" || k == "pulsarcentric" ||
+                 k == "nsamples") {
+        int v;
+        rd_i(v);
+      } else if (k == "src_raj" || k == "src_dej" || k == "az_start" || k == "za_start" ||
+                 k == "refdm" || k == "period") {
+        double v;
+        rd_d(v);
+      } else {
+        throw py::value_error("unsupported filterbank header keyword '" + k + "'");
+      }
+      if (!f) throw py::value_error("truncated filterbank header: " + filename);
+    }
+    if (nchans <= 0 || (nbits != 8 && nbits != 16 && nbits != 32))
+      throw py::value_error("unsupported filterbank layout (nchans/nbits)");
+    if (nifs != 1) throw py::value_error("only nifs == 1 is supported");
+    const std::streampos start = f.tellg();
+    f.seekg(0, std::ios::end);
+    const size_t bytes = (size_t)(f.tellg() - start);
+    const size_t row = (size_t)nchans * nifs * (size_t)(nbits / 8);
+    ndata = (long)(bytes / row);
+    raw = std::make_shared<std::vector<uint8_t>>((size_t)ndata * row);
+    f.seekg(start);
+    f.read((char *)raw->data(), (std::streamsize)raw->size());
+    if (foff < 0) {
+      const size_t es = (size_t)(nbits / 8);
+      for (long t = 0; t < ndata; ++t) {
+        uint8_t *r = raw->data() + (size_t)t * row;
+        for (int a = 0, b = nchans - 1; a < b; ++a, --b)
+          for (size_t e = 0; e < es; ++e) std::swap(r[a * es + e], r[b * es + e]);
+      }
+      fch1 = fch1 + (nchans - 1) * foff;
+      foff = -foff;
+    }
+  }
+
+  Header header() const {
+    return Header((float)tstart, (float)tsamp, (float)fch1, (float)foff, nbits, nchans, ndata,
+                  filename);
+  }
+
+  py::array data() const {
+    const std::vector<py::ssize_t> shape{ndata, nifs, nchans};
+    auto keep = new std::shared_ptr<std::vector<uint8_t>>(raw);
+    py::capsule cap(keep, [](void *p) { delete static_cast<std::shared_ptr<std::vector<uint8_t>> *>(p); });
+    switch (nbits) {
+    case 8: return py::array_t<uint8_t>(shape, (const uint8_t *)raw->data(), cap);
+    case 16: return py::array_t<uint16_t>(shape, (const uint16_t *)raw->data(), cap);
+    default: return py::array_t<uint32_t>(shape, (const uint32_t *)raw->data(), cap);
+    }
+  }
+};
+
+DedispersedDataUint8 dedispered_fil(std::string filename, float dm_low, float dm_high,
+                                    float freq_start, float freq_end, float dm_step,
+                                    int time_downsample, float t_sample, int target, int target_id,
+                                    std::string mask_file, RFIConfig rficfg) {
+  if (target != 1)
+    throw std::runtime_error("target 0 (the CPU dedispersion) is not part of this library; "
+                             "use target=1");
+  Filterbank fil(filename);
+  const Header h = fil.header();
+  py::array a = fil.data();
+  switch (fil.nbits) {
+  case 8:
+    return dedisperse_spec<uint8_t>(a.cast<py::array_t<uint8_t, py::array::c_style>>(), h, dm_low,
+                                    dm_high, freq_start, freq_end, dm_step, time_downsample,
+                                    t_sample, target_id, mask_file, rficfg);
+  case 16:
+    return dedisperse_spec<uint16_t>(a.cast<py::array_t<uint16_t, py::array::c_style>>(), h,
+                                     dm_low, dm_high, freq_start, freq_end, dm_step,
+                                     time_downsample, t_sample, target_id, mask_file, rficfg);
+  default:
+    return dedisperse_spec<uint32_t>(a.cast<py::array_t<uint32_t, py::array::c_style>>(), h,
+                                     dm_low, dm_high, freq_start, freq_end, dm_step,
+                                     time_downsample, t_sample, target_id, mask_file, rficfg);
+  }
+}
+
+template <typename T>
+void def_spec(py::module_ &m) {
+  m.def("_dedisperse_spec", &dedisperse_spec<T>, py::arg("data"), py::arg("header"),
+        py::arg("dm_low"), py::arg("dm_high"), py::arg("freq_start"), py::arg("freq_end"),
+        py::arg("dm_step") = 1.0f, py::arg("time_downsample") = 2, py::arg("t_sample") = 0.5f,
+        py::arg("target_id") = 0, py::arg("mask_file") = "mask.txt",
+        py::arg("rficfg") = RFIConfig());
+  m.def("_dedisperse_spec_raw", &dedisperse_spec_raw<T>, py::arg("data"), py::arg("header"),
+        py::arg("dm_low"), py::arg("dm_high"), py::arg("freq_start"), py::arg("freq_end"),
+        py::arg("dm_step") = 1.0f, py::arg("time_downsample") = 1, py::arg("target_id") = 0,
+        py::arg("mask_file") = "mask.txt", py::arg("rficfg") = RFIConfig(),
+        py::arg("return_masks") = false);
+}
+
+template <typename D>
+void def_result(py::module_ &m, const char *name) {
+  py::class_<D>(m, name)
+      .def(py::init<>())
+      .def_readonly("dm_times", &D::dm_times)
+      .def_readonly("shape", &D::shape)
+      .def_readonly("dm_ndata", &D::dm_ndata)
+      .def_readonly("downtsample_ndata", &D::downtsample_ndata)
+      .def_readonly("dm_low", &D::dm_low)
+      .def_readonly("dm_high", &D::dm_high)
+      .def_readonly("dm_step", &D::dm_step)
+      .def_readonly("tsample", &D::tsample)
+      .def_readonly("filname", &D::filname);
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_astroflow_core, m) {
+  m.doc() = "astroflow dedispersion core on MI355X (gfx950), over libastroflow_amd";
+
+  def_result<DedispersedDataUint8>(m, "DedispersedDataUint8");
+  def_result<DedisperedData>(m, "DedisperedData");
+
+  py::class_<Header>(m, "Header")
+      .def(py::init<>())
+      .def(py::init<float, float, float, float, int, int, long, std::string>())
+      .def(py::init<py::object>())
+      .def_readonly("mjd", &Header::mjd)
+      .def_readonly("tsamp", &Header::tsamp)
+      .def_readonly("fch1", &Header::fch1)
+      .def_readonly("foff", &Header::foff)
+      .def_readonly("nbits", &Header::nbits)
+      .def_readonly("nchans", &Header::nchans)
+      .def_readonly("ndata", &Header::ndata)
+      .def_readonly("filename", &Header::filename);
+
+  py::class_<IQRMConfig>(m, "IQRMconfig")
+      .def(py::init<>())
+      .def(py::init<int, float, float, double, double, double, bool>())
+      .def(py::init<py::object>())
+      .def_readonly("mode", &IQRMConfig::mode)
+      .def_readonly("radius_frac", &IQRMConfig::radius_frac)
+      .def_readonly("nsigma", &IQRMConfig::nsigma)
+      .def_readonly("geofactor", &IQRMConfig::geofactor)
+      .def_readonly("win_sec", &IQRMConfig::win_sec)
+      .def_readonly("hop_sec", &IQRMConfig::hop_sec)
+      .def_readonly("include_tail", &IQRMConfig::include_tail);
+
+  py::class_<RFIConfig>(m, "RFIconfig")
+      .def(py::init<>())
+      .def(py::init<bool, bool, bool, IQRMConfig>())
+      .def(py::init<py::object>())
+      .def_readonly("use_zero_dm", &RFIConfig::use_zero_dm)
+      .def_readonly("use_mask", &RFIConfig::use_mask)
+      .def_readonly("use_iqrm", &RFIConfig::use_iqrm)
+      .def_readonly("iqrm_cfg", &RFIConfig::iqrm_cfg);
+
+  py::class_<Filterbank>(m, "Filterbank")
+      .def(py::init<>())
+      .def(py::init<const std::string &>())
+      .def_readonly("filename", &Filterbank::filename)
+      .def_readonly("nchans", &Filterbank::nchans)
+      .def_readonly("nifs", &Filterbank::nifs)
+      .def_readonly("nbits", &Filterbank::nbits)
+      .def_readonly("fch1", &Filterbank::fch1)
+      .def_readonly("foff", &Filterbank::foff)
+      .def_readonly("tstart", &Filterbank::tstart)
+      .def_readonly("tsamp", &Filterbank::tsamp)
+      .def_readonly("ndata", &Filterbank::ndata)
+      .def_property_readonly("data", &Filterbank::data)
+      .def_property_readonly("header", &Filterbank::header);
+
+  // float32 first: pybind11 tries exact dtype matches before converting, and
+  // a float array must not be silently converted to an integer overload
+  def_spec<float>(m);
+  def_spec<uint8_t>(m);
+  def_spec<uint16_t>(m);
+  def_spec<uint32_t>(m);
+
+  m.def("_dedispered_fil", &dedispered_fil, py::arg("filename"), py::arg("dm_low"),
+        py::arg("dm_high"), py::arg("freq_start"), py::arg("freq_end"), py::arg("dm_step"),
+        py::arg("time_downsample"), py::arg("t_sample"), py::arg("target") = 1,
+        py::arg("target_id") = 0, py::arg("maskfile") = "mask.txt",
+        py::arg("rficfg") = RFIConfig());
+
+  m.def("_image_slice", [](py::array_t<float, py::array::c_style> plane, int size) {
+    // one DM-time image from a float plane (exposed for tests of the imaging)
+    if (plane.ndim() != 2) throw py::value_error("plane must be 2-D");
+    py::array_t<uint8_t> out({size, size, 3});
+    afimg::slice_image(plane.data(), (size_t)plane.shape(1), 0, (int)plane.shape(0),
+                       (int)plane.shape(1), size, out.mutable_data());
+    return out;
+  }, py::arg("plane"), py::arg("size") = 512);
+}

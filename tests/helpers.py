@@ -29,3 +29,25 @@ def make_spec(x: np.ndarray, band: synth.Band, dm_low, dm_step, ndm, **kw) -> ca
 def small_band(nchan, f_lo=1000.0, f_hi=1500.0, tsamp=49.152e-6):
     fch1, foff = synth.ascending_header(f_lo, f_hi, nchan)
     return synth.Band(nchan, fch1, foff, np.float32(tsamp))
+
+
+def write_sigproc(path, data: np.ndarray, fch1: float, foff: float, tsamp: float,
+                  tstart: float = 60000.0, source="synthetic"):
+    """A SIGPROC filterbank file: keyword header, then [nsamp][nchan] samples."""
+    import struct
+
+    def s(x):
+        b = x.encode()
+        return struct.pack("<i", len(b)) + b
+
+    nbits = data.dtype.itemsize * 8
+    hdr = s("HEADER_START") + s("source_name") + s(source)
+    hdr += s("machine_id") + struct.pack("<i", 0) + s("telescope_id") + struct.pack("<i", 0)
+    hdr += s("data_type") + struct.pack("<i", 1)
+    hdr += s("fch1") + struct.pack("<d", fch1) + s("foff") + struct.pack("<d", foff)
+    hdr += s("nchans") + struct.pack("<i", data.shape[1]) + s("nbits") + struct.pack("<i", nbits)
+    hdr += s("tstart") + struct.pack("<d", tstart) + s("tsamp") + struct.pack("<d", tsamp)
+    hdr += s("nifs") + struct.pack("<i", 1) + s("HEADER_END")
+    with open(path, "wb") as f:
+        f.write(hdr)
+        f.write(np.ascontiguousarray(data).tobytes())
