@@ -44,7 +44,7 @@ class PlanDesc(C.Structure):
         ("iqrm_include_tail", C.c_int),
         ("use_zero_dm", C.c_int), ("zdm_c0", C.c_int), ("zdm_c1", C.c_int),
         ("bad_chans", C.POINTER(C.c_int)), ("n_bad", C.c_int),
-        ("tile_t", C.c_int), ("dm_per_block", C.c_int),
+        ("tile_t", C.c_int), ("dm_per_block", C.c_int), ("inter_budget", C.c_size_t),
     ]
 
 
@@ -118,6 +118,12 @@ SIGNATURES = {
     "af_stream_sync": (_I, [_P]),
     "af_sync": (_I, []),
     "af_time_execute": (_I, [_P, _P, _P, _I, C.POINTER(C.c_float)]),
+    "af_batch_create": (_I, [C.POINTER(_P), _I, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
+                             C.POINTER(_P)]),
+    "af_batch_execute": (_I, [_P, _I, _P]),
+    "af_batch_info": (_I, [_P, C.POINTER(_I), C.POINTER(_I)]),
+    "af_batch_destroy": (_I, [_P]),
+    "af_autotune": (_I, [C.POINTER(PlanDesc), _P, _P, _I, _I, C.POINTER(C.c_float)]),
     "af_host_alloc": (_I, [C.POINTER(_P), _S]),
     "af_host_free": (_I, [_P]),
     "af_server_create": (_I, [C.POINTER(ServerDesc), C.POINTER(_P)]),
@@ -208,6 +214,7 @@ class DedispSpec:
     bad_channels: List[int] = field(default_factory=list)
     tile_t: int = 0
     dm_per_block: int = 0
+    inter_budget: int = 0
 
     def to_desc(self):
         """Returns (PlanDesc, keepalive) -- keepalive owns the arrays it points to."""
@@ -236,6 +243,7 @@ class DedispSpec:
         d.bad_chans = bad.ctypes.data_as(C.POINTER(C.c_int)) if len(bad) else None
         d.n_bad = len(bad)
         d.tile_t, d.dm_per_block = self.tile_t, self.dm_per_block
+        d.inter_budget = self.inter_budget
         return d, (freq, bad)
 
 
@@ -439,3 +447,52 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Owns an af_batch over (plan, input, output[, masks]) device buffers."""
+
+    def __init__(self, plans, d_ins, d_outs, d_masks=None):
+        n = len(plans)
+        arr = lambda vals: (C.c_void_p * n)(*[C.c_void_p(v or None) for v in vals])
+        ph = (C.c_void_p * n)(*[p.handle for p in plans])
+        h = C.c_void_p()
+        check(lib().af_batch_create(ph, n, arr(d_ins), arr(d_outs),
+                                    arr(d_masks) if d_masks is not None else None, C.byref(h)),
+              "af_batch_create")
+        self._h = h
+        self.plans = list(plans)
+
+    def execute(self, use_graph: bool = False, stream: int = 0):
+        check(lib().af_batch_execute(self._h, int(use_graph),
+                                     C.c_void_p(stream) if stream else None), "af_batch_execute")
+
+    def info(self):
+        a, b = C.c_int(), C.c_int()
+        check(lib().af_batch_info(self._h, C.byref(a), C.byref(b)))
+        return {"launches": a.value, "recorded": b.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().af_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(spec: "DedispSpec", d_in: int, d_out: int, tune_subbands: bool = False,
+             reps: int = 2):
+    """Fastest (tile_t, dm_per_block, sbw) for the spec on these buffers;
+    returns (tuned spec copy, ms per execute)."""
+    import dataclasses
+    desc, keep = spec.to_desc()
+    ms = C.c_float(0)
+    check(lib().af_autotune(C.byref(desc), C.c_void_p(d_in), C.c_void_p(d_out),
+                            int(tune_subbands), reps, C.byref(ms)), "af_autotune")
+    tuned = dataclasses.replace(spec, tile_t=desc.tile_t, dm_per_block=desc.dm_per_block,
+                                sbw=desc.sb_nchan)
+    return tuned, ms.value

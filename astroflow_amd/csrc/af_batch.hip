@@ -1,0 +1,320 @@
+// af_batch.hip -- many plans (beams, chunks) run as one launch per stage.
+//
+// The reference runs one chunk per call, each stage a separate launch
+// (src/core/dedispered.cu:988-1463).  A batch records the launches every
+// plan's execute would make (af_execute with a recorder set), then merges
+// the launches of the same stage and kernel across plans into one: the
+// merged launch carries a table of argument blocks and the prefix sums of
+// their workgroup counts, and each workgroup finds its plan by a binary
+// search over that prefix (af_kernels*.hip, AF_BATCHED).  Stages stay
+// ordered (bin, IQRM statistics, reduction, lags, vote, segment masks,
+// clean, dedispersion stage 1 and 2, mask copies), so every plan sees the
+// same dependencies as in its own execute; results are identical.  The
+// launch sequence can be captured once into a hipGraph and replayed.
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "af_internal.h"
+
+namespace af {
+namespace {
+
+struct Merged {
+  int kind = 0;
+  bool batched = false;
+  LaunchDesc single;              // when not batched
+  size_t tab_off = 0, start_off = 0;
+  int n = 0;
+  uint32_t total = 0, lds = 0;
+};
+
+}  // namespace
+}  // namespace af
+
+struct af_batch {
+  int device = 0;
+  std::vector<af_plan *> plans;
+  std::vector<af::Merged> launches;
+  int recorded = 0;               // launches the plans would make one by one
+  uint8_t *d_tables = nullptr;
+  hipStream_t cap = nullptr;      // capture stream
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace af {
+namespace {
+
+af_status run_launches(af_batch *b, hipStream_t st) {
+  for (const Merged &m : b->launches) {
+    if (m.batched)
+      AF_HIP(launch_batched(m.kind, b->d_tables + m.tab_off,
+                            (const uint32_t *)(b->d_tables + m.start_off), m.n, m.total, m.lds, st));
+    else
+      AF_HIP(launch_desc(m.single, st));
+  }
+  return AF_OK;
+}
+
+void release(af_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->cap) (void)hipStreamDestroy(b->cap);
+  if (b->d_tables) (void)hipFree(b->d_tables);
+}
+
+}  // namespace
+}  // namespace af
+
+using namespace af;
+
+extern "C" {
+
+af_status af_batch_create(af_plan *const *plans, int nplans, const void *const *d_ins,
+                          void *const *d_outs, uint8_t *const *d_masks, af_batch **batch) {
+  if (!plans || nplans <= 0 || !d_ins || !d_outs || !batch)
+    return fail(AF_ERR_INVALID, "null argument");
+  *batch = nullptr;
+  for (int i = 0; i < nplans; ++i)
+    if (!plans[i] || !d_ins[i] || !d_outs[i]) return fail(AF_ERR_INVALID, "null plan or buffer");
+  af_batch *b = new af_batch();
+  b->plans.assign(plans, plans + nplans);
+  (void)hipGetDevice(&b->device);
+  // 1. every plan's launches, in its own order
+  std::vector<std::vector<LaunchDesc>> recs(nplans);
+  for (int i = 0; i < nplans; ++i) {
+    af_plan *p = plans[i];
+    if (p->device != b->device) {
+      delete b;
+      return fail(AF_ERR_INVALID, "all plans of a batch must be on the current device");
+    }
+    const bool prof = p->profiling;
+    p->profiling = false;
+    g_record = &recs[i];
+    af_status s = execute(p, d_ins[i], d_outs[i], d_masks ? d_masks[i] : nullptr, nullptr);
+    g_record = nullptr;
+    p->profiling = prof;
+    if (s != AF_OK) {
+      delete b;
+      return s;
+    }
+    b->recorded += (int)recs[i].size();
+  }
+  // 2. merge by (stage, kind), stages in order, kinds in order of appearance.
+  // A plan whose output runs in several time super-tiles launches stage 1
+  // and stage 2 once per super-tile, reusing its intermediate buffer: those
+  // launches stay single and in the plan's own order (placed after the
+  // merged stage-2 launches; plans are independent of each other).
+  std::vector<uint8_t> host;
+  auto put = [&](const void *src, size_t n) {
+    size_t off = (host.size() + 255) / 256 * 256;
+    host.resize(off + n);
+    std::memcpy(host.data() + off, src, n);
+    return off;
+  };
+  std::vector<bool> tiled(nplans, false);
+  for (int i = 0; i < nplans; ++i) {
+    int n7 = 0;
+    for (auto &d : recs[i]) n7 += d.stage == 7;
+    tiled[i] = n7 > 1;
+  }
+  auto emit_group = [&](int stage, int kind) -> af_status {
+    std::vector<const LaunchDesc *> group;
+    for (int i = 0; i < nplans; ++i)
+      for (auto &d : recs[i])
+        if (d.stage == stage && d.kind == kind && !(tiled[i] && (stage == 7 || stage == 8)))
+          group.push_back(&d);
+    if (group.empty()) return AF_OK;
+    if (!batchable(kind) || group.size() < 2) {
+      for (auto *d : group) {
+        Merged m;
+        m.kind = kind;
+        m.single = *d;
+        b->launches.push_back(m);
+      }
+      return AF_OK;
+    }
+    Merged m;
+    m.kind = kind;
+    m.batched = true;
+    m.n = (int)group.size();
+    std::vector<uint8_t> tab;
+    std::vector<uint32_t> start(1, 0);
+    for (auto *d : group) {
+      tab.insert(tab.end(), d->args, d->args + d->args_size);
+      const uint64_t nx = (uint64_t)start.back() + d->nblk;
+      if (nx > 0x7FFFFFFFull) return fail(AF_ERR_INVALID, "batched launch exceeds 2^31 workgroups");
+      start.push_back((uint32_t)nx);
+      m.lds = std::max(m.lds, d->lds);
+    }
+    m.total = start.back();
+    m.tab_off = put(tab.data(), tab.size());
+    m.start_off = put(start.data(), start.size() * sizeof(uint32_t));
+    b->launches.push_back(m);
+    return AF_OK;
+  };
+  for (int stage = 0; stage <= 9; ++stage) {
+    std::vector<int> kinds;
+    for (auto &r : recs)
+      for (auto &d : r)
+        if (d.stage == stage && std::find(kinds.begin(), kinds.end(), d.kind) == kinds.end())
+          kinds.push_back(d.kind);
+    for (int kind : kinds) {
+      af_status s = emit_group(stage, kind);
+      if (s != AF_OK) {
+        release(b);
+        delete b;
+        return s;
+      }
+    }
+    if (stage == 8)
+      for (int i = 0; i < nplans; ++i)
+        if (tiled[i])
+          for (auto &d : recs[i])
+            if (d.stage == 7 || d.stage == 8) {
+              Merged m;
+              m.kind = d.kind;
+              m.single = d;
+              b->launches.push_back(m);
+            }
+  }
+  if (!host.empty()) {
+    if (hipMalloc(&b->d_tables, host.size()) != hipSuccess) {
+      release(b);
+      delete b;
+      return fail(AF_ERR_NOMEM, "hipMalloc batch tables");
+    }
+    if (hipMemcpy(b->d_tables, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      release(b);
+      delete b;
+      return fail(AF_ERR_RUNTIME, "upload of batch tables failed");
+    }
+  }
+  *batch = b;
+  return AF_OK;
+}
+
+af_status af_batch_execute(af_batch *b, int use_graph, void *stream) {
+  if (!b) return fail(AF_ERR_INVALID, "null argument");
+  AF_HIP(hipSetDevice(b->device));
+  hipStream_t st = (hipStream_t)stream;
+  if (!use_graph) return run_launches(b, st);
+  if (!b->exec) {
+    AF_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+    AF_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+    af_status s = run_launches(b, b->cap);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->cap, &g);
+    if (s != AF_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return s;
+    }
+    if (e != hipSuccess) return fail(AF_ERR_RUNTIME, std::string("graph capture: ") + hipGetErrorString(e));
+    b->graph = g;
+    AF_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  AF_HIP(hipGraphLaunch(b->exec, st));
+  return AF_OK;
+}
+
+af_status af_batch_info(const af_batch *b, int *launches, int *recorded) {
+  if (!b) return fail(AF_ERR_INVALID, "null argument");
+  if (launches) *launches = (int)b->launches.size();
+  if (recorded) *recorded = b->recorded;
+  return AF_OK;
+}
+
+af_status af_batch_destroy(af_batch *b) {
+  if (!b) return AF_OK;
+  (void)hipSetDevice(b->device);
+  (void)hipDeviceSynchronize();
+  release(b);
+  delete b;
+  return AF_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ autotuner --
+namespace af {
+namespace {
+
+struct TuneKey {
+  int dtype, nchan, ds, dd_c0, dd_c1, ndm, method, sb_nchan, use_iqrm, tune_sb;
+  size_t nsamp, nout;
+  double dm_step;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(dtype, nchan, ds, dd_c0, dd_c1, ndm, method, sb_nchan, use_iqrm, tune_sb, nsamp,
+                    nout, dm_step) < std::tie(o.dtype, o.nchan, o.ds, o.dd_c0, o.dd_c1, o.ndm,
+                                              o.method, o.sb_nchan, o.use_iqrm, o.tune_sb,
+                                              o.nsamp, o.nout, o.dm_step);
+  }
+};
+
+struct TuneVal {
+  int tile_t, dm_per_block, sb_nchan;
+  float ms;
+};
+
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneVal> g_tune;
+
+}  // namespace
+}  // namespace af
+
+extern "C" af_status af_autotune(af_plan_desc *desc, const void *d_in, void *d_out,
+                                 int tune_subbands, int reps, float *best_ms) {
+  if (!desc || !d_in || !d_out) return fail(AF_ERR_INVALID, "null argument");
+  if (reps < 1) reps = 1;
+  const TuneKey key{desc->dtype, desc->nchan, desc->ds, desc->dd_c0, desc->dd_c1, desc->ndm,
+                    desc->method, tune_subbands ? 0 : desc->sb_nchan, desc->use_iqrm,
+                    tune_subbands, desc->nsamp, desc->nout, desc->dm_step};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      desc->tile_t = it->second.tile_t;
+      desc->dm_per_block = it->second.dm_per_block;
+      desc->sb_nchan = it->second.sb_nchan;
+      if (best_ms) *best_ms = it->second.ms;
+      return AF_OK;
+    }
+  }
+  const bool u8 = desc->dtype == AF_U8;
+  const bool sub = desc->method == AF_METHOD_SUBBAND;
+  std::vector<int> dpbs = {0};
+  if (u8) dpbs.push_back(16);
+  std::vector<int> tiles = {0};
+  if (u8 && sub) tiles = {0, 256, 512, 1024};
+  std::vector<int> sbs = {desc->sb_nchan};
+  if (sub && tune_subbands) sbs = {16, 32, 64, 128};
+  TuneVal best{desc->tile_t, desc->dm_per_block, desc->sb_nchan, 1e30f};
+  for (int sb : sbs)
+    for (int dpb : dpbs)
+      for (int tile : tiles) {
+        af_plan_desc d = *desc;
+        d.sb_nchan = sb;
+        d.dm_per_block = dpb;
+        d.tile_t = tile;
+        af_plan *p = nullptr;
+        if (af_plan_create(&d, &p) != AF_OK) continue;   // a shape the kernels cannot take
+        float ms = 0;
+        af_status s = af_time_execute(p, d_in, d_out, 1, &ms);   // warm-up
+        if (s == AF_OK) s = af_time_execute(p, d_in, d_out, reps, &ms);
+        af_plan_destroy(p);
+        if (s != AF_OK) continue;
+        ms /= (float)reps;
+        if (ms < best.ms) best = TuneVal{tile, dpb, sb, ms};
+      }
+  if (best.ms >= 1e29f) return fail(AF_ERR_INVALID, "no candidate plan could be built");
+  desc->tile_t = best.tile_t;
+  desc->dm_per_block = best.dm_per_block;
+  desc->sb_nchan = best.sb_nchan;
+  if (best_ms) *best_ms = best.ms;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune[key] = best;
+  return AF_OK;
+}

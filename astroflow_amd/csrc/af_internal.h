@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -37,6 +38,7 @@ constexpr int kDdTile = 64 * kDdLanesOut;     // outputs per wave tile
 constexpr int kDdRowsPerBlock = 4;            // DM rows (waves) per workgroup
 constexpr int kStatRows = 1024;               // rows per statistics item
 constexpr int kCleanRows = 64;                // rows per cleaning item
+constexpr size_t kInterBudget = size_t(1) << 30;   // subband intermediate per super-tile
 
 size_t dtype_size(int dtype);
 int acc_size(int dtype);  // bytes of one output element
@@ -127,13 +129,118 @@ struct Stage2Args {
   const int32_t *res;       // [ndm][nsb]
   const int32_t *maxres_j;  // [ndm_nom]
   int ndm, ndm0, ndm_nom;
-  void *out;                // [ndm][nout]
-  size_t nout;
+  void *out;                // [ndm][ostride], this call's first output column
+  size_t nout;              // output columns written per row by this call
+  size_t ostride;           // row stride of out
   int tile;                 // outputs per workgroup tile
   int lds_len;              // LDS row length (tile + max over groups of maxres)
   int inter16;              // inter is u16 (u8 input)
 };
 hipError_t launch_stage2(int dtype, const Stage2Args &a, hipStream_t st);
+// Device-to-device copy on `st` (recorded like the kernels, see below).
+hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st);
+
+// Kernel argument blocks of the launchers that take scalar arguments.
+struct BinArgs {
+  const void *in;
+  void *out;
+  size_t nsamp, nbin;
+  int nchan, ds;
+};
+struct StatArgs {
+  const void *x;
+  const RowItem *items;
+  void *part;               // Part* (generic) or uint2* (u8)
+  int nchan, c0, c1, nunits;
+};
+struct ReduceArgs {
+  const void *part;         // Part* (generic) or uint2* (u8)
+  const int *seg_items;
+  Part *segsum;
+  int nseg, Csub;
+};
+struct LagArgs {
+  const Part *segsum;
+  const uint32_t *wseg, *wt0, *wt1;
+  const int *lags;
+  uint8_t *flags;
+  int Csub, mode, nlags;
+  float nsigma;
+};
+struct MergeArgs {
+  const uint8_t *flags;
+  const int *lags;
+  uint8_t *masks;
+  int nlags, Csub, c0, nchan;
+};
+struct SegmaskArgs {
+  const uint8_t *masks;
+  const Seg *segs;
+  uint8_t *segmask;
+  int nseg, nchan;
+};
+struct CopyArgs {
+  void *dst;
+  const void *src;
+  size_t bytes;
+};
+
+// Every launch the library makes, as data: which kernel (kind), its grid,
+// workgroup size, dynamic LDS and argument block.  Used to merge the same
+// stage of many plans into one launch (af_batch).
+enum LaunchKind : int {
+  KB_BIN_U8, KB_BIN_U16, KB_BIN_U32, KB_BIN_F32,
+  KB_STAT_U8, KB_STAT_U16, KB_STAT_U32, KB_STAT_F32,
+  KB_RED_U8, KB_RED_U16, KB_RED_U32, KB_RED_F32,
+  KB_LAG_U8, KB_LAG_U16, KB_LAG_U32, KB_LAG_F32,
+  KB_MERGE, KB_SEGMASK,
+  KB_CLEAN_U8, KB_CLEAN_U16, KB_CLEAN_U32, KB_CLEAN_F32,
+  KB_DD_U8Q, KB_DD_U8, KB_DD_U8LDS, KB_DD_G_U8, KB_DD_G_U16, KB_DD_G_U32, KB_DD_G_F32F,
+  KB_DD_G_F32D,
+  KB_S2_V8, KB_S2_V16, KB_S2_U16, KB_S2_G_U32, KB_S2_G_U64, KB_S2_G_F32,
+  KB_COPY,
+  KB_COUNT
+};
+
+struct LaunchDesc {
+  int kind = 0;
+  int stage = 0;            // pipeline position: bin 0 .. mask copy 9
+  uint32_t nblk = 0, threads = 0, lds = 0;
+  uint32_t args_size = 0;
+  alignas(16) unsigned char args[192];
+};
+
+// While set (per thread), the launchers append a LaunchDesc here instead of
+// launching.
+extern thread_local std::vector<LaunchDesc> *g_record;
+
+// Records (when a recorder is set) or launches one described kernel.
+hipError_t emit_desc(const LaunchDesc &d, hipStream_t st);
+
+template <typename A>
+inline hipError_t emit_args(int kind, int stage, size_t nblk, unsigned threads, size_t lds,
+                            const A &args, hipStream_t st) {
+  static_assert(sizeof(A) <= sizeof(LaunchDesc::args), "argument block too large");
+  if (nblk == 0) return hipSuccess;
+  if (nblk > 0x7FFFFFFFull) return hipErrorInvalidValue;
+  LaunchDesc d;
+  d.kind = kind;
+  d.stage = stage;
+  d.nblk = (uint32_t)nblk;
+  d.threads = threads;
+  d.lds = (uint32_t)lds;
+  d.args_size = (uint32_t)sizeof(A);
+  std::memcpy(d.args, &args, sizeof(A));
+  return emit_desc(d, st);
+}
+
+// One launch of `kind` covering n argument blocks: workgroup b belongs to the
+// block i with start[i] <= b < start[i+1] (tables in device memory).
+hipError_t launch_batched(int kind, const void *d_tab, const uint32_t *d_start, int n,
+                          uint32_t total_blocks, uint32_t lds, hipStream_t st);
+// A recorded launch, launched on its own.
+hipError_t launch_desc(const LaunchDesc &d, hipStream_t st);
+bool batchable(int kind);
 
 }  // namespace af
 
@@ -146,7 +253,9 @@ struct af_plan {
 
   size_t nbin = 0, nout = 0, Lp = 0, L1 = 0;
   int nch = 0, ndm = 0, nsb = 0, ndm_nom = 0, maxdelay = 0, maxres = 0;
-  int ntiles1 = 0;          // tiles of the first dedispersion stage
+  int ntiles1 = 0;          // tiles of the first dedispersion stage (per super-tile)
+  size_t T2 = 0;            // outputs per time super-tile (subband method)
+  int nsuper = 1;           // super-tiles
   int inter_elem = 0;       // bytes per subband partial sum
   int s2_tile = 0, s2_lds = 0, maxlds = 0;
   int dd_lds_rows = 0, dd_lds_win = 0;   // LDS-staged u8 dedispersion

@@ -68,11 +68,14 @@ __host__ __device__ inline double u128_to_double(uint64_t hi, uint64_t lo) {
 // ------------------------------------------------------------------ binning --
 // src/core/dedispered.cu:337-376 (sqrt compression of the ds-sample sum).
 template <typename T>
-__global__ void k_bin(const T *__restrict__ in, T *__restrict__ out, size_t nsamp,
-                      int nchan, int ds, size_t nbin) {
+__device__ __forceinline__ void bin_body(const BinArgs &a, uint32_t bid, uint32_t nblk) {
+  const T *__restrict__ in = (const T *)a.in;
+  T *__restrict__ out = (T *)a.out;
+  const size_t nsamp = a.nsamp, nbin = a.nbin;
+  const int nchan = a.nchan, ds = a.ds;
   const size_t total = nbin * (size_t)nchan;
-  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-       idx += (size_t)gridDim.x * blockDim.x) {
+  for (size_t idx = (size_t)bid * blockDim.x + threadIdx.x; idx < total;
+       idx += (size_t)nblk * blockDim.x) {
     const size_t c = idx % (size_t)nchan, s = idx / (size_t)nchan;
     const size_t t0 = s * (size_t)ds;
     const size_t t1 = t0 + (size_t)ds < nsamp ? t0 + (size_t)ds : nsamp;
@@ -94,19 +97,23 @@ __global__ void k_bin(const T *__restrict__ in, T *__restrict__ out, size_t nsam
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_bin(BinArgs a) { bin_body<T>(a, blockIdx.x, gridDim.x); }
+
 // ------------------------------------------------------- IQRM statistics --
 // Per (row block, channel) sums over the block's rows: the first step of
 // src/core/iqrm.cu:21-69, with exact integer sums (coalesced: a wave reads 64
 // consecutive channels of a row).
 template <typename T>
-__global__ __launch_bounds__(256) void k_seg_stats(const T *__restrict__ x, int nchan,
-                                                   int c0, int c1,
-                                                   const RowItem *__restrict__ items,
-                                                   Part *__restrict__ part) {
+__device__ __forceinline__ void stats_body(const StatArgs &a, uint32_t bid, uint32_t) {
+  const T *__restrict__ x = (const T *)a.x;
+  const int nchan = a.nchan, c0 = a.c0, c1 = a.c1;
+  const RowItem *__restrict__ items = a.items;
+  Part *__restrict__ part = (Part *)a.part;
   const int Csub = c1 - c0;
   const int nchunk = (Csub + 255) / 256;
-  const int item = blockIdx.x / nchunk;
-  const int c = c0 + (blockIdx.x % nchunk) * 256 + threadIdx.x;
+  const int item = (int)bid / nchunk;
+  const int c = c0 + ((int)bid % nchunk) * 256 + threadIdx.x;
   if (c >= c1) return;
   const RowItem it = items[item];
   Part p{0, 0, 0};
@@ -135,11 +142,19 @@ __global__ __launch_bounds__(256) void k_seg_stats(const T *__restrict__ x, int 
   part[(size_t)item * Csub + (c - c0)] = p;
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_seg_stats(StatArgs a) {
+  stats_body<T>(a, blockIdx.x, gridDim.x);
+}
+
 // Sum the row-block partials of each segment in item order.
 template <typename T>
-__global__ void k_seg_reduce(const Part *__restrict__ part, const int *__restrict__ seg_items,
-                             int nseg, int Csub, Part *__restrict__ segsum) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void reduce_body(const ReduceArgs &a, uint32_t bid, uint32_t) {
+  const Part *__restrict__ part = (const Part *)a.part;
+  const int *__restrict__ seg_items = a.seg_items;
+  const int nseg = a.nseg, Csub = a.Csub;
+  Part *__restrict__ segsum = a.segsum;
+  const size_t idx = (size_t)bid * blockDim.x + threadIdx.x;
   if (idx >= (size_t)nseg * Csub) return;
   const int s = (int)(idx / Csub), c = (int)(idx % Csub);
   Part acc{0, 0, 0};
@@ -163,19 +178,28 @@ __global__ void k_seg_reduce(const Part *__restrict__ part, const int *__restric
   segsum[(size_t)s * Csub + c] = acc;
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_seg_reduce(ReduceArgs a) {
+  reduce_body<T>(a, blockIdx.x, gridDim.x);
+}
+
 // --------------------------------------------------- IQRM robust voting --
 // One workgroup per (window, lag): statistic from the window's segment sums
 // (src/core/iqrm.cu:72-91), lagged difference with edge clipping (:94-117),
 // quartiles by a bitonic sort in LDS and linear interpolation in double
 // (:288-306), and the flag test (:120-135).
 template <typename T>
-__global__ __launch_bounds__(1024) void k_iqrm_lag(
-    const Part *__restrict__ segsum, const uint32_t *__restrict__ wseg,
-    const uint32_t *__restrict__ wt0, const uint32_t *__restrict__ wt1, int Csub,
-    int mode, const int *__restrict__ lags, int nlags, float nsigma,
-    uint8_t *__restrict__ flags) {
+__device__ __forceinline__ void lag_body(const LagArgs &a, uint32_t bid, uint32_t) {
+  const Part *__restrict__ segsum = a.segsum;
+  const uint32_t *__restrict__ wseg = a.wseg;
+  const uint32_t *__restrict__ wt0 = a.wt0;
+  const uint32_t *__restrict__ wt1 = a.wt1;
+  const int Csub = a.Csub, mode = a.mode, nlags = a.nlags;
+  const int *__restrict__ lags = a.lags;
+  const float nsigma = a.nsigma;
+  uint8_t *__restrict__ flags = a.flags;
   extern __shared__ float lds[];
-  const int w = blockIdx.x / nlags, li = blockIdx.x % nlags;
+  const int w = (int)bid / nlags, li = (int)bid % nlags;
   const int lag = lags[li];
   int P2 = 1;
   while (P2 < Csub) P2 <<= 1;
@@ -289,18 +313,24 @@ __global__ __launch_bounds__(1024) void k_iqrm_lag(
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(1024) void k_iqrm_lag(LagArgs a) {
+  lag_body<T>(a, blockIdx.x, gridDim.x);
+}
+
 __device__ __forceinline__ int iqrm_partner(int k, int lag, int C) {
   if (lag < 0) return (k < C + lag) ? k - lag : C - 1;
   return (k < lag) ? 0 : k - lag;
 }
 
 // Votes and final window mask (src/core/iqrm.cu:138-185).
-__global__ __launch_bounds__(1024) void k_iqrm_merge(const uint8_t *__restrict__ flags,
-                                                     const int *__restrict__ lags,
-                                                     int nlags, int Csub, int c0,
-                                                     int nchan, uint8_t *__restrict__ masks) {
+__device__ __forceinline__ void merge_body(const MergeArgs &a, uint32_t bid, uint32_t) {
+  const uint8_t *__restrict__ flags = a.flags;
+  const int *__restrict__ lags = a.lags;
+  const int nlags = a.nlags, Csub = a.Csub, c0 = a.c0, nchan = a.nchan;
+  uint8_t *__restrict__ masks = a.masks;
   extern __shared__ int cast[];
-  const int w = blockIdx.x;
+  const int w = (int)bid;
   const uint8_t *f = flags + (size_t)w * nlags * Csub;
   for (int k = threadIdx.x; k < Csub; k += blockDim.x) cast[k] = 0;
   __syncthreads();
@@ -326,15 +356,26 @@ __global__ __launch_bounds__(1024) void k_iqrm_merge(const uint8_t *__restrict__
   }
 }
 
+__global__ __launch_bounds__(1024) void k_iqrm_merge(MergeArgs a) {
+  merge_body(a, blockIdx.x, gridDim.x);
+}
+
 // OR of the masks of the windows covering each segment.
-__global__ void k_segmask(const uint8_t *__restrict__ masks, const Seg *__restrict__ segs,
-                          int nseg, int nchan, uint8_t *__restrict__ segmask) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void segmask_body(const SegmaskArgs &a, uint32_t bid, uint32_t) {
+  const uint8_t *__restrict__ masks = a.masks;
+  const Seg *__restrict__ segs = a.segs;
+  const int nseg = a.nseg, nchan = a.nchan;
+  uint8_t *__restrict__ segmask = a.segmask;
+  const size_t idx = (size_t)bid * blockDim.x + threadIdx.x;
   if (idx >= (size_t)nseg * nchan) return;
   const int s = (int)(idx / nchan), c = (int)(idx % nchan);
   uint8_t m = 0;
   for (uint32_t w = segs[s].w0; w < segs[s].w1; ++w) m |= masks[(size_t)w * nchan + c];
   segmask[idx] = m;
+}
+
+__global__ __launch_bounds__(256) void k_segmask(SegmaskArgs a) {
+  segmask_body(a, blockIdx.x, gridDim.x);
 }
 
 // --------------------------------------------- clean + transpose pass --
@@ -357,10 +398,10 @@ __device__ __forceinline__ T zdm_sub(T v, float m) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_clean(CleanArgs a) {
+__device__ __forceinline__ void clean_body(const CleanArgs &a, uint32_t bid, uint32_t) {
   __shared__ float m_s[kCleanRows];
   __shared__ T tile[64][kCleanRows + 4];
-  const RowItem it = a.items[blockIdx.x];
+  const RowItem it = a.items[bid];
   const int nrows = (int)(it.r1 - it.r0);
   const T *__restrict__ x = (const T *)a.x;
   T *__restrict__ xT = (T *)a.xT;
@@ -419,6 +460,11 @@ __global__ __launch_bounds__(256) void k_clean(CleanArgs a) {
     }
     __syncthreads();
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_clean(CleanArgs a) {
+  clean_body<T>(a, blockIdx.x, gridDim.x);
 }
 
 // ------------------------------------------------------- dedispersion --
@@ -480,9 +526,8 @@ __device__ __forceinline__ void dd_class(const uint8_t *__restrict__ tb, size_t 
 // u8 dedispersion over quarter-class-sorted channel lists (same tiling and
 // packed-lane accumulation as k_dd_u8; channel blocks of <= 256 channels,
 // each summed in the four classes and flushed).
-__global__ __launch_bounds__(256) void k_dd_u8_q(DdArgs a) {
+__device__ __forceinline__ void dd_u8_q_body(const DdArgs &a, uint32_t lid, uint32_t) {
   const int nrowblk = (a.nrows + kDdRowsPerBlock - 1) / kDdRowsPerBlock;
-  const uint32_t lid = xcd_remap(blockIdx.x, gridDim.x);
   int rowblk, tile, grp;
   if (a.worklist) {
     const int4 wi = a.worklist[lid];
@@ -554,9 +599,12 @@ __global__ __launch_bounds__(256) void k_dd_u8_q(DdArgs a) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_dd_u8(DdArgs a) {
+__global__ __launch_bounds__(256) void k_dd_u8_q(DdArgs a) {
+  dd_u8_q_body(a, xcd_remap(blockIdx.x, gridDim.x), gridDim.x);
+}
+
+__device__ __forceinline__ void dd_u8_body(const DdArgs &a, uint32_t lid, uint32_t) {
   const int nrowblk = (a.nrows + kDdRowsPerBlock - 1) / kDdRowsPerBlock;
-  const uint32_t lid = xcd_remap(blockIdx.x, gridDim.x);
   int rowblk, tile, grp;
   if (a.worklist) {
     // host-sorted order: workgroups that read overlapping input run together
@@ -639,6 +687,10 @@ __global__ __launch_bounds__(256) void k_dd_u8(DdArgs a) {
         if (t + i < a.nvalid) o[i] = tot[i];
     }
   }
+}
+
+__global__ __launch_bounds__(256) void k_dd_u8(DdArgs a) {
+  dd_u8_body(a, xcd_remap(blockIdx.x, gridDim.x), gridDim.x);
 }
 
 // u8 dedispersion with the delay windows staged in LDS.  A workgroup owns JB
@@ -770,9 +822,8 @@ __global__ __launch_bounds__(JB * 64) void k_dd_u8_lds(DdArgs a) {
 // Element-wise dedispersion for u16/u32/f32 data (accumulator Acc, output
 // OutT), same tiling as k_dd_u8.
 template <typename T, typename Acc, typename OutT>
-__global__ __launch_bounds__(256) void k_dd_generic(DdArgs a) {
+__device__ __forceinline__ void dd_generic_body(const DdArgs &a, uint32_t lid, uint32_t) {
   const int nrowblk = (a.nrows + kDdRowsPerBlock - 1) / kDdRowsPerBlock;
-  const uint32_t lid = xcd_remap(blockIdx.x, gridDim.x);
   const int rowblk = (int)(lid % (uint32_t)nrowblk);
   const uint32_t rest = lid / (uint32_t)nrowblk;
   const int tile = (int)(rest % (uint32_t)a.ntiles);
@@ -799,14 +850,18 @@ __global__ __launch_bounds__(256) void k_dd_generic(DdArgs a) {
     if (t + i < a.nvalid) o[i] = (OutT)acc[i];
 }
 
+template <typename T, typename Acc, typename OutT>
+__global__ __launch_bounds__(256) void k_dd_generic(DdArgs a) {
+  dd_generic_body<T, Acc, OutT>(a, xcd_remap(blockIdx.x, gridDim.x), gridDim.x);
+}
+
 // Subband stage 2 for u16 partial sums (src/core/dedispered.cu:449-476):
 // out[m][t] = sum_sb inter[j][sb][t + res[m][sb]], j = m / ndm0.  A workgroup
 // stages the group's partial sums for its tile (tile + max residual of the
 // group) in LDS, then each wave sums the subbands for one fine DM at a time.
-__global__ __launch_bounds__(256) void k_stage2_u16(Stage2Args a) {
+__device__ __forceinline__ void stage2_u16_body(const Stage2Args &a, uint32_t lid, uint32_t) {
   extern __shared__ uint16_t s2[];
   const int ntile = (int)((a.nout + a.tile - 1) / a.tile);
-  const uint32_t lid = xcd_remap(blockIdx.x, gridDim.x);
   const int ti = (int)(lid % (uint32_t)ntile);
   const int j = (int)(lid / (uint32_t)ntile);
   const size_t t0 = (size_t)ti * a.tile;
@@ -832,16 +887,20 @@ __global__ __launch_bounds__(256) void k_stage2_u16(Stage2Args a) {
       for (int i = 0; i < 16; ++i)
         if (i < per) acc[i] += p[i];
     }
-    uint32_t *o = (uint32_t *)a.out + (size_t)m * a.nout + t0 + (size_t)lane * per;
+    uint32_t *o = (uint32_t *)a.out + (size_t)m * a.ostride + t0 + (size_t)lane * per;
     for (int i = 0; i < per; ++i)
       if (t0 + (size_t)lane * per + i < a.nout) o[i] = acc[i];
   }
 }
 
+__global__ __launch_bounds__(256) void k_stage2_u16(Stage2Args a) {
+  stage2_u16_body(a, xcd_remap(blockIdx.x, gridDim.x), gridDim.x);
+}
+
 // Stage 2 for u32/u64/double partial sums, straight from HBM/L2.
 template <typename Acc, typename OutT>
-__global__ __launch_bounds__(256) void k_stage2_generic(Stage2Args a) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void stage2_generic_body(const Stage2Args &a, uint32_t bid, uint32_t) {
+  const size_t idx = (size_t)bid * blockDim.x + threadIdx.x;
   if (idx >= (size_t)a.ndm * a.nout) return;
   const int m = (int)(idx / a.nout);
   const size_t t = idx % a.nout;
@@ -849,10 +908,73 @@ __global__ __launch_bounds__(256) void k_stage2_generic(Stage2Args a) {
   const Acc *src = (const Acc *)a.inter + (size_t)j * a.nsb * a.L1 + t;
   Acc acc = 0;
   for (int sb = 0; sb < a.nsb; ++sb) acc += src[(size_t)sb * a.L1 + a.res[(size_t)m * a.nsb + sb]];
-  ((OutT *)a.out)[idx] = (OutT)acc;
+  ((OutT *)a.out)[(size_t)m * a.ostride + t] = (OutT)acc;
 }
 
+template <typename Acc, typename OutT>
+__global__ __launch_bounds__(256) void k_stage2_generic(Stage2Args a) {
+  stage2_generic_body<Acc, OutT>(a, blockIdx.x, gridDim.x);
+}
+
+// ------------------------------------------------------ batched launches --
+// One launch runs the same stage of many plans: workgroup g (after the XCD
+// remap) belongs to argument block i with start[i] <= g < start[i+1] and runs
+// as that block's workgroup g - start[i].
+__device__ __forceinline__ int batch_find(const uint32_t *__restrict__ start, int n, uint32_t g) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+#define AF_BATCHED(NAME, ARGS, BOUNDS, CALL)                                            \
+  __global__ __launch_bounds__(BOUNDS) void NAME(const ARGS *__restrict__ tab,          \
+                                                 const uint32_t *__restrict__ start, int n) { \
+    const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);                                \
+    const int i = batch_find(start, n, g);                                              \
+    const ARGS a = tab[i];                                                              \
+    const uint32_t lid = g - start[i], nb = start[i + 1] - start[i];                    \
+    (void)nb;                                                                           \
+    CALL;                                                                               \
+  }
+
+AF_BATCHED(kb_bin_u8, BinArgs, 256, (bin_body<uint8_t>(a, lid, nb)))
+AF_BATCHED(kb_bin_u16, BinArgs, 256, (bin_body<uint16_t>(a, lid, nb)))
+AF_BATCHED(kb_bin_u32, BinArgs, 256, (bin_body<uint32_t>(a, lid, nb)))
+AF_BATCHED(kb_bin_f32, BinArgs, 256, (bin_body<float>(a, lid, nb)))
+AF_BATCHED(kb_stat_u16, StatArgs, 256, (stats_body<uint16_t>(a, lid, nb)))
+AF_BATCHED(kb_stat_u32, StatArgs, 256, (stats_body<uint32_t>(a, lid, nb)))
+AF_BATCHED(kb_stat_f32, StatArgs, 256, (stats_body<float>(a, lid, nb)))
+AF_BATCHED(kb_red_u16, ReduceArgs, 256, (reduce_body<uint16_t>(a, lid, nb)))
+AF_BATCHED(kb_red_u32, ReduceArgs, 256, (reduce_body<uint32_t>(a, lid, nb)))
+AF_BATCHED(kb_red_f32, ReduceArgs, 256, (reduce_body<float>(a, lid, nb)))
+AF_BATCHED(kb_lag_u8, LagArgs, 1024, (lag_body<uint8_t>(a, lid, nb)))
+AF_BATCHED(kb_lag_u16, LagArgs, 1024, (lag_body<uint16_t>(a, lid, nb)))
+AF_BATCHED(kb_lag_u32, LagArgs, 1024, (lag_body<uint32_t>(a, lid, nb)))
+AF_BATCHED(kb_lag_f32, LagArgs, 1024, (lag_body<float>(a, lid, nb)))
+AF_BATCHED(kb_merge, MergeArgs, 1024, (merge_body(a, lid, nb)))
+AF_BATCHED(kb_segmask, SegmaskArgs, 256, (segmask_body(a, lid, nb)))
+AF_BATCHED(kb_clean_u16, CleanArgs, 256, (clean_body<uint16_t>(a, lid, nb)))
+AF_BATCHED(kb_clean_u32, CleanArgs, 256, (clean_body<uint32_t>(a, lid, nb)))
+AF_BATCHED(kb_clean_f32, CleanArgs, 256, (clean_body<float>(a, lid, nb)))
+AF_BATCHED(kb_dd_u8q, DdArgs, 256, (dd_u8_q_body(a, lid, nb)))
+AF_BATCHED(kb_dd_u8, DdArgs, 256, (dd_u8_body(a, lid, nb)))
+AF_BATCHED(kb_dd_g_u8, DdArgs, 256, (dd_generic_body<uint8_t, uint32_t, uint32_t>(a, lid, nb)))
+AF_BATCHED(kb_dd_g_u16, DdArgs, 256, (dd_generic_body<uint16_t, uint32_t, uint32_t>(a, lid, nb)))
+AF_BATCHED(kb_dd_g_u32, DdArgs, 256, (dd_generic_body<uint32_t, uint64_t, uint64_t>(a, lid, nb)))
+AF_BATCHED(kb_dd_g_f32f, DdArgs, 256, (dd_generic_body<float, double, float>(a, lid, nb)))
+AF_BATCHED(kb_dd_g_f32d, DdArgs, 256, (dd_generic_body<float, double, double>(a, lid, nb)))
+AF_BATCHED(kb_s2_u16, Stage2Args, 256, (stage2_u16_body(a, lid, nb)))
+AF_BATCHED(kb_s2_g_u32, Stage2Args, 256, (stage2_generic_body<uint32_t, uint32_t>(a, lid, nb)))
+AF_BATCHED(kb_s2_g_u64, Stage2Args, 256, (stage2_generic_body<uint64_t, uint64_t>(a, lid, nb)))
+AF_BATCHED(kb_s2_g_f32, Stage2Args, 256, (stage2_generic_body<double, float>(a, lid, nb)))
+
 // ---------------------------------------------------------- launchers --
+
+thread_local std::vector<LaunchDesc> *g_record = nullptr;
 
 static unsigned grid_for(size_t n, unsigned bs, unsigned cap = 1u << 20) {
   size_t g = (n + bs - 1) / bs;
@@ -860,44 +982,48 @@ static unsigned grid_for(size_t n, unsigned bs, unsigned cap = 1u << 20) {
   return (unsigned)(g < cap ? g : cap);
 }
 
-#define AF_DISPATCH(dtype, T, ...)                                            \
-  switch (dtype) {                                                            \
-  case AF_U8: { using T = uint8_t; __VA_ARGS__; break; }                      \
-  case AF_U16: { using T = uint16_t; __VA_ARGS__; break; }                    \
-  case AF_U32: { using T = uint32_t; __VA_ARGS__; break; }                    \
-  case AF_F32: { using T = float; __VA_ARGS__; break; }                       \
-  default: return hipErrorInvalidValue;                                       \
+hipError_t emit_desc(const LaunchDesc &d, hipStream_t st) {
+  if (g_record) {
+    g_record->push_back(d);
+    return hipSuccess;
   }
+  return launch_desc(d, st);
+}
+
+template <typename A>
+static hipError_t emit(int kind, int stage, size_t nblk, unsigned threads, size_t lds, const A &args,
+                       hipStream_t st) {
+  return emit_args(kind, stage, nblk, threads, lds, args, st);
+}
+
+#define AF_KIND4(base, dtype)                                                    \
+  ((dtype) == AF_U8 ? base##U8 : (dtype) == AF_U16 ? base##U16 : (dtype) == AF_U32 ? base##U32 \
+                                                                                 : base##F32)
 
 hipError_t launch_bin(int dtype, const void *in, void *out, size_t nsamp, int nchan,
                       int ds, size_t nbin, hipStream_t st) {
-  const size_t n = nbin * (size_t)nchan;
-  AF_DISPATCH(dtype, T,
-              hipLaunchKernelGGL(k_bin<T>, dim3(grid_for(n, 256)), dim3(256), 0, st,
-                                 (const T *)in, (T *)out, nsamp, nchan, ds, nbin));
-  return hipGetLastError();
+  if (dtype < AF_U8 || dtype > AF_F32) return hipErrorInvalidValue;
+  BinArgs a{in, out, nsamp, nbin, nchan, ds};
+  return emit(AF_KIND4(KB_BIN_, dtype), 0, grid_for(nbin * (size_t)nchan, 256), 256, 0, a, st);
 }
 
 hipError_t launch_seg_stats(int dtype, const void *x, int nchan, int c0, int c1,
                             const RowItem *items, int nitems, Part *part, hipStream_t st) {
   if (dtype == AF_U8)
     return launch_seg_stats_u8((const uint8_t *)x, nchan, c0, c1, items, nitems, (uint2 *)part, st);
+  if (dtype < AF_U8 || dtype > AF_F32) return hipErrorInvalidValue;
   const int nchunk = (c1 - c0 + 255) / 256;
-  AF_DISPATCH(dtype, T,
-              hipLaunchKernelGGL(k_seg_stats<T>, dim3((unsigned)(nitems * nchunk)), dim3(256),
-                                 0, st, (const T *)x, nchan, c0, c1, items, part));
-  return hipGetLastError();
+  StatArgs a{x, items, part, nchan, c0, c1, nitems * nchunk};
+  return emit(AF_KIND4(KB_STAT_, dtype), 1, (size_t)nitems * nchunk, 256, 0, a, st);
 }
 
 hipError_t launch_seg_reduce(int dtype, const Part *part, const int *seg_items, int nseg,
                              int Csub, Part *segsum, hipStream_t st) {
   if (dtype == AF_U8)
     return launch_seg_reduce_u8((const uint2 *)part, seg_items, nseg, Csub, segsum, st);
-  const size_t n = (size_t)nseg * Csub;
-  AF_DISPATCH(dtype, T,
-              hipLaunchKernelGGL(k_seg_reduce<T>, dim3((unsigned)((n + 255) / 256)), dim3(256),
-                                 0, st, part, seg_items, nseg, Csub, segsum));
-  return hipGetLastError();
+  if (dtype < AF_U8 || dtype > AF_F32) return hipErrorInvalidValue;
+  ReduceArgs a{part, seg_items, segsum, nseg, Csub};
+  return emit(AF_KIND4(KB_RED_, dtype), 2, ((size_t)nseg * Csub + 255) / 256, 256, 0, a, st);
 }
 
 hipError_t launch_iqrm_lags(int dtype, const Part *segsum, const Seg *segs,
@@ -905,98 +1031,194 @@ hipError_t launch_iqrm_lags(int dtype, const Part *segsum, const Seg *segs,
                             int nwin, int Csub, int mode, const int *lags, int nlags,
                             float nsigma, uint8_t *flags, size_t lds_bytes, hipStream_t st) {
   (void)segs;
-  AF_DISPATCH(dtype, T,
-              hipLaunchKernelGGL(k_iqrm_lag<T>, dim3((unsigned)(nwin * nlags)), dim3(1024),
-                                 lds_bytes, st, segsum, wseg, wt0, wt1, Csub, mode, lags,
-                                 nlags, nsigma, flags));
-  return hipGetLastError();
+  if (dtype < AF_U8 || dtype > AF_F32) return hipErrorInvalidValue;
+  LagArgs a{segsum, wseg, wt0, wt1, lags, flags, Csub, mode, nlags, nsigma};
+  return emit(AF_KIND4(KB_LAG_, dtype), 3, (size_t)nwin * nlags, 1024, lds_bytes, a, st);
 }
 
 hipError_t launch_iqrm_merge(const uint8_t *flags, const int *lags, int nlags, int nwin,
                              int Csub, int c0, int nchan, uint8_t *masks, hipStream_t st) {
-  hipLaunchKernelGGL(k_iqrm_merge, dim3((unsigned)nwin), dim3(1024),
-                     (size_t)Csub * sizeof(int), st, flags, lags, nlags, Csub, c0, nchan, masks);
-  return hipGetLastError();
+  MergeArgs a{flags, lags, masks, nlags, Csub, c0, nchan};
+  return emit(KB_MERGE, 4, (size_t)nwin, 1024, (size_t)Csub * sizeof(int), a, st);
 }
 
 hipError_t launch_segmask(const uint8_t *masks, const Seg *segs, int nseg, int nchan,
                           uint8_t *segmask, hipStream_t st) {
-  const size_t n = (size_t)nseg * nchan;
-  hipLaunchKernelGGL(k_segmask, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, masks,
-                     segs, nseg, nchan, segmask);
-  return hipGetLastError();
+  SegmaskArgs a{masks, segs, segmask, nseg, nchan};
+  return emit(KB_SEGMASK, 5, ((size_t)nseg * nchan + 255) / 256, 256, 0, a, st);
 }
 
 hipError_t launch_clean(int dtype, const CleanArgs &a, hipStream_t st) {
   if (dtype == AF_U8 && a.nchan <= kCleanU8MaxChan) return launch_clean_u8(a, st);
-  AF_DISPATCH(dtype, T,
-              hipLaunchKernelGGL(k_clean<T>, dim3((unsigned)a.nitems), dim3(256), 0, st, a));
-  return hipGetLastError();
+  if (dtype < AF_U8 || dtype > AF_F32) return hipErrorInvalidValue;
+  return emit(AF_KIND4(KB_CLEAN_, dtype), 6, (size_t)a.nitems, 256, 0, a, st);
 }
 
 hipError_t launch_dedisperse(int dtype, const DdArgs &a, hipStream_t st) {
   const int nrowblk = (a.nrows + kDdRowsPerBlock - 1) / kDdRowsPerBlock;
   const size_t nblk = (size_t)nrowblk * a.ntiles * a.ngrp;
   if (nblk == 0) return hipSuccess;
-  if (nblk > 0x7FFFFFFFull) return hipErrorInvalidValue;
-  const dim3 g((unsigned)nblk), b(256);
   switch (dtype) {
   case AF_U8:
     if (a.lds_rows == 16 && (a.out16 || a.ngrp == 1)) {
       const size_t nblk16 = (size_t)((a.nrows + 15) / 16) * a.ntiles * a.ngrp;
       const size_t lds = 2 * (size_t)1024 * 16;   // two buffers of one 16-byte chunk per thread
-      hipLaunchKernelGGL(k_dd_u8_lds<16>, dim3((unsigned)nblk16), dim3(1024), lds, st, a);
-    } else if (a.chan_order && (a.out16 || a.ngrp == 1)) {
-      hipLaunchKernelGGL(k_dd_u8_q, g, b, 0, st, a);
-    } else if (a.out16 || a.ngrp == 1) {
-      hipLaunchKernelGGL(k_dd_u8, g, b, 0, st, a);
-    } else {
-      hipLaunchKernelGGL((k_dd_generic<uint8_t, uint32_t, uint32_t>), g, b, 0, st, a);
+      return emit(KB_DD_U8LDS, 7, nblk16, 1024, lds, a, st);
     }
+    if (a.chan_order && (a.out16 || a.ngrp == 1)) return emit(KB_DD_U8Q, 7, nblk, 256, 0, a, st);
+    if (a.out16 || a.ngrp == 1) return emit(KB_DD_U8, 7, nblk, 256, 0, a, st);
+    return emit(KB_DD_G_U8, 7, nblk, 256, 0, a, st);
+  case AF_U16: return emit(KB_DD_G_U16, 7, nblk, 256, 0, a, st);
+  case AF_U32: return emit(KB_DD_G_U32, 7, nblk, 256, 0, a, st);
+  case AF_F32: return emit(a.ngrp == 1 ? KB_DD_G_F32F : KB_DD_G_F32D, 7, nblk, 256, 0, a, st);
+  default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_stage2(int dtype, const Stage2Args &a, hipStream_t st) {
+  if (dtype == AF_U8 && a.inter16 && (a.tile == 512 || a.tile == 1024)) {
+    const int ntile = (int)((a.nout + a.tile - 1) / a.tile);
+    const size_t lds = (size_t)a.nsb * a.lds_len * sizeof(uint16_t);
+    return emit(a.tile == 1024 ? KB_S2_V16 : KB_S2_V8, 8, (size_t)ntile * a.ndm_nom, 512, lds, a,
+                st);
+  }
+  if (dtype == AF_U8 && a.inter16) {
+    const int ntile = (int)((a.nout + a.tile - 1) / a.tile);
+    const size_t lds = (size_t)a.nsb * a.lds_len * sizeof(uint16_t);
+    return emit(KB_S2_U16, 8, (size_t)ntile * a.ndm_nom, 256, lds, a, st);
+  }
+  const size_t n = (size_t)a.ndm * a.nout;
+  const size_t g = (n + 255) / 256;
+  switch (dtype) {
+  case AF_U8:
+  case AF_U16: return emit(KB_S2_G_U32, 8, g, 256, 0, a, st);
+  case AF_U32: return emit(KB_S2_G_U64, 8, g, 256, 0, a, st);
+  case AF_F32: return emit(KB_S2_G_F32, 8, g, 256, 0, a, st);
+  default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st) {
+  if (bytes == 0) return hipSuccess;
+  CopyArgs a{dst, src, bytes};
+  return emit(KB_COPY, 9, 1, 1, 0, a, st);
+}
+
+// u8 kernels live in af_kernels_u8.hip
+hipError_t launch_desc_u8(const LaunchDesc &d, hipStream_t st);
+hipError_t launch_batched_u8(int kind, const void *tab, const uint32_t *start, int n,
+                             uint32_t total, uint32_t lds, hipStream_t st);
+
+template <typename A>
+static const A &argsof(const LaunchDesc &d) {
+  return *reinterpret_cast<const A *>(d.args);
+}
+
+hipError_t launch_desc(const LaunchDesc &d, hipStream_t st) {
+  const dim3 g(d.nblk), b(d.threads);
+  switch (d.kind) {
+  case KB_BIN_U8: hipLaunchKernelGGL(k_bin<uint8_t>, g, b, d.lds, st, argsof<BinArgs>(d)); break;
+  case KB_BIN_U16: hipLaunchKernelGGL(k_bin<uint16_t>, g, b, d.lds, st, argsof<BinArgs>(d)); break;
+  case KB_BIN_U32: hipLaunchKernelGGL(k_bin<uint32_t>, g, b, d.lds, st, argsof<BinArgs>(d)); break;
+  case KB_BIN_F32: hipLaunchKernelGGL(k_bin<float>, g, b, d.lds, st, argsof<BinArgs>(d)); break;
+  case KB_STAT_U16: hipLaunchKernelGGL(k_seg_stats<uint16_t>, g, b, d.lds, st, argsof<StatArgs>(d)); break;
+  case KB_STAT_U32: hipLaunchKernelGGL(k_seg_stats<uint32_t>, g, b, d.lds, st, argsof<StatArgs>(d)); break;
+  case KB_STAT_F32: hipLaunchKernelGGL(k_seg_stats<float>, g, b, d.lds, st, argsof<StatArgs>(d)); break;
+  case KB_RED_U16: hipLaunchKernelGGL(k_seg_reduce<uint16_t>, g, b, d.lds, st, argsof<ReduceArgs>(d)); break;
+  case KB_RED_U32: hipLaunchKernelGGL(k_seg_reduce<uint32_t>, g, b, d.lds, st, argsof<ReduceArgs>(d)); break;
+  case KB_RED_F32: hipLaunchKernelGGL(k_seg_reduce<float>, g, b, d.lds, st, argsof<ReduceArgs>(d)); break;
+  case KB_LAG_U8: hipLaunchKernelGGL(k_iqrm_lag<uint8_t>, g, b, d.lds, st, argsof<LagArgs>(d)); break;
+  case KB_LAG_U16: hipLaunchKernelGGL(k_iqrm_lag<uint16_t>, g, b, d.lds, st, argsof<LagArgs>(d)); break;
+  case KB_LAG_U32: hipLaunchKernelGGL(k_iqrm_lag<uint32_t>, g, b, d.lds, st, argsof<LagArgs>(d)); break;
+  case KB_LAG_F32: hipLaunchKernelGGL(k_iqrm_lag<float>, g, b, d.lds, st, argsof<LagArgs>(d)); break;
+  case KB_MERGE: hipLaunchKernelGGL(k_iqrm_merge, g, b, d.lds, st, argsof<MergeArgs>(d)); break;
+  case KB_SEGMASK: hipLaunchKernelGGL(k_segmask, g, b, d.lds, st, argsof<SegmaskArgs>(d)); break;
+  case KB_CLEAN_U16: hipLaunchKernelGGL(k_clean<uint16_t>, g, b, d.lds, st, argsof<CleanArgs>(d)); break;
+  case KB_CLEAN_U32: hipLaunchKernelGGL(k_clean<uint32_t>, g, b, d.lds, st, argsof<CleanArgs>(d)); break;
+  case KB_CLEAN_F32: hipLaunchKernelGGL(k_clean<float>, g, b, d.lds, st, argsof<CleanArgs>(d)); break;
+  case KB_DD_U8Q: hipLaunchKernelGGL(k_dd_u8_q, g, b, d.lds, st, argsof<DdArgs>(d)); break;
+  case KB_DD_U8: hipLaunchKernelGGL(k_dd_u8, g, b, d.lds, st, argsof<DdArgs>(d)); break;
+  case KB_DD_U8LDS: hipLaunchKernelGGL(k_dd_u8_lds<16>, g, b, d.lds, st, argsof<DdArgs>(d)); break;
+  case KB_DD_G_U8:
+    hipLaunchKernelGGL((k_dd_generic<uint8_t, uint32_t, uint32_t>), g, b, d.lds, st, argsof<DdArgs>(d));
     break;
-  case AF_U16:
-    hipLaunchKernelGGL((k_dd_generic<uint16_t, uint32_t, uint32_t>), g, b, 0, st, a);
+  case KB_DD_G_U16:
+    hipLaunchKernelGGL((k_dd_generic<uint16_t, uint32_t, uint32_t>), g, b, d.lds, st, argsof<DdArgs>(d));
     break;
-  case AF_U32:
-    hipLaunchKernelGGL((k_dd_generic<uint32_t, uint64_t, uint64_t>), g, b, 0, st, a);
+  case KB_DD_G_U32:
+    hipLaunchKernelGGL((k_dd_generic<uint32_t, uint64_t, uint64_t>), g, b, d.lds, st, argsof<DdArgs>(d));
     break;
-  case AF_F32:
-    if (a.ngrp == 1)
-      hipLaunchKernelGGL((k_dd_generic<float, double, float>), g, b, 0, st, a);
-    else
-      hipLaunchKernelGGL((k_dd_generic<float, double, double>), g, b, 0, st, a);
+  case KB_DD_G_F32F:
+    hipLaunchKernelGGL((k_dd_generic<float, double, float>), g, b, d.lds, st, argsof<DdArgs>(d));
     break;
+  case KB_DD_G_F32D:
+    hipLaunchKernelGGL((k_dd_generic<float, double, double>), g, b, d.lds, st, argsof<DdArgs>(d));
+    break;
+  case KB_S2_U16: hipLaunchKernelGGL(k_stage2_u16, g, b, d.lds, st, argsof<Stage2Args>(d)); break;
+  case KB_S2_G_U32:
+    hipLaunchKernelGGL((k_stage2_generic<uint32_t, uint32_t>), g, b, d.lds, st, argsof<Stage2Args>(d));
+    break;
+  case KB_S2_G_U64:
+    hipLaunchKernelGGL((k_stage2_generic<uint64_t, uint64_t>), g, b, d.lds, st, argsof<Stage2Args>(d));
+    break;
+  case KB_S2_G_F32:
+    hipLaunchKernelGGL((k_stage2_generic<double, float>), g, b, d.lds, st, argsof<Stage2Args>(d));
+    break;
+  case KB_COPY: {
+    const CopyArgs &c = argsof<CopyArgs>(d);
+    return hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToDevice, st);
+  }
   default:
-    return hipErrorInvalidValue;
+    return launch_desc_u8(d, st);
   }
   return hipGetLastError();
 }
 
-hipError_t launch_stage2(int dtype, const Stage2Args &a, hipStream_t st) {
-  if (dtype == AF_U8 && a.inter16 && (a.tile == 512 || a.tile == 1024))
-    return launch_stage2_u16v(a, st);
-  if (dtype == AF_U8 && a.inter16) {
-    const int ntile = (int)((a.nout + a.tile - 1) / a.tile);
-    const size_t nblk = (size_t)ntile * a.ndm_nom;
-    const size_t lds = (size_t)a.nsb * a.lds_len * sizeof(uint16_t);
-    hipLaunchKernelGGL(k_stage2_u16, dim3((unsigned)nblk), dim3(256), lds, st, a);
-    return hipGetLastError();
-  }
-  const size_t n = (size_t)a.ndm * a.nout;
-  const dim3 g((unsigned)((n + 255) / 256)), b(256);
-  switch (dtype) {
-  case AF_U8:
-  case AF_U16:
-    hipLaunchKernelGGL((k_stage2_generic<uint32_t, uint32_t>), g, b, 0, st, a);
+bool batchable(int kind) {
+  return kind != KB_DD_U8LDS && kind != KB_COPY && kind >= 0 && kind < KB_COUNT;
+}
+
+hipError_t launch_batched(int kind, const void *tab, const uint32_t *start, int n,
+                          uint32_t total, uint32_t lds, hipStream_t st) {
+  const dim3 g(total);
+  switch (kind) {
+#define AF_BL(K, KERN, TH, A)                                                        \
+  case K:                                                                            \
+    hipLaunchKernelGGL(KERN, g, dim3(TH), lds, st, (const A *)tab, start, n);        \
     break;
-  case AF_U32:
-    hipLaunchKernelGGL((k_stage2_generic<uint64_t, uint64_t>), g, b, 0, st, a);
-    break;
-  case AF_F32:
-    hipLaunchKernelGGL((k_stage2_generic<double, float>), g, b, 0, st, a);
-    break;
+    AF_BL(KB_BIN_U8, kb_bin_u8, 256, BinArgs)
+    AF_BL(KB_BIN_U16, kb_bin_u16, 256, BinArgs)
+    AF_BL(KB_BIN_U32, kb_bin_u32, 256, BinArgs)
+    AF_BL(KB_BIN_F32, kb_bin_f32, 256, BinArgs)
+    AF_BL(KB_STAT_U16, kb_stat_u16, 256, StatArgs)
+    AF_BL(KB_STAT_U32, kb_stat_u32, 256, StatArgs)
+    AF_BL(KB_STAT_F32, kb_stat_f32, 256, StatArgs)
+    AF_BL(KB_RED_U16, kb_red_u16, 256, ReduceArgs)
+    AF_BL(KB_RED_U32, kb_red_u32, 256, ReduceArgs)
+    AF_BL(KB_RED_F32, kb_red_f32, 256, ReduceArgs)
+    AF_BL(KB_LAG_U8, kb_lag_u8, 1024, LagArgs)
+    AF_BL(KB_LAG_U16, kb_lag_u16, 1024, LagArgs)
+    AF_BL(KB_LAG_U32, kb_lag_u32, 1024, LagArgs)
+    AF_BL(KB_LAG_F32, kb_lag_f32, 1024, LagArgs)
+    AF_BL(KB_MERGE, kb_merge, 1024, MergeArgs)
+    AF_BL(KB_SEGMASK, kb_segmask, 256, SegmaskArgs)
+    AF_BL(KB_CLEAN_U16, kb_clean_u16, 256, CleanArgs)
+    AF_BL(KB_CLEAN_U32, kb_clean_u32, 256, CleanArgs)
+    AF_BL(KB_CLEAN_F32, kb_clean_f32, 256, CleanArgs)
+    AF_BL(KB_DD_U8Q, kb_dd_u8q, 256, DdArgs)
+    AF_BL(KB_DD_U8, kb_dd_u8, 256, DdArgs)
+    AF_BL(KB_DD_G_U8, kb_dd_g_u8, 256, DdArgs)
+    AF_BL(KB_DD_G_U16, kb_dd_g_u16, 256, DdArgs)
+    AF_BL(KB_DD_G_U32, kb_dd_g_u32, 256, DdArgs)
+    AF_BL(KB_DD_G_F32F, kb_dd_g_f32f, 256, DdArgs)
+    AF_BL(KB_DD_G_F32D, kb_dd_g_f32d, 256, DdArgs)
+    AF_BL(KB_S2_U16, kb_s2_u16, 256, Stage2Args)
+    AF_BL(KB_S2_G_U32, kb_s2_g_u32, 256, Stage2Args)
+    AF_BL(KB_S2_G_U64, kb_s2_g_u64, 256, Stage2Args)
+    AF_BL(KB_S2_G_F32, kb_s2_g_f32, 256, Stage2Args)
+#undef AF_BL
   default:
-    return hipErrorInvalidValue;
+    return launch_batched_u8(kind, tab, start, n, total, lds, st);
   }
   return hipGetLastError();
 }

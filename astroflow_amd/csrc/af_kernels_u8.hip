@@ -43,16 +43,17 @@ __device__ __forceinline__ uint4 load_row16(const uint8_t *row, int cb, int lim)
 // LDS.  Partials: u32 sum (<= 1024*255) and u32 square sum (<= 1024*65025).
 constexpr int kStatThreads = 1024;
 
-__global__ __launch_bounds__(1024) void k_seg_stats_u8(const uint8_t *__restrict__ x, int nchan,
-                                                       int c0, int c1,
-                                                       const RowItem *__restrict__ items,
-                                                       uint2 *__restrict__ part, int nunits) {
+__device__ __forceinline__ void stats_u8_body(const StatArgs &sa, uint32_t bid, uint32_t nblk) {
+  const uint8_t *__restrict__ x = (const uint8_t *)sa.x;
+  const int nchan = sa.nchan, c0 = sa.c0, c1 = sa.c1, nunits = sa.nunits;
+  const RowItem *__restrict__ items = sa.items;
+  uint2 *__restrict__ part = (uint2 *)sa.part;
   __shared__ uint32_t ls[4096], lq[4096];
   const int Csub = c1 - c0;
   const int ngrp = (Csub + 15) / 16;
   const int nchunk = (ngrp + 255) / 256;
   // persistent: a grid sized to the machine walks all (item, chunk) units
-  for (int unit = blockIdx.x; unit < nunits; unit += gridDim.x) {
+  for (int unit = (int)bid; unit < nunits; unit += (int)nblk) {
   const int item = unit / nchunk;
   const int gl = threadIdx.x & 255;
   const int rl = threadIdx.x >> 8;
@@ -125,16 +126,21 @@ __global__ __launch_bounds__(1024) void k_seg_stats_u8(const uint8_t *__restrict
   }
 }
 
+__global__ __launch_bounds__(1024) void k_seg_stats_u8(StatArgs a) {
+  stats_u8_body(a, blockIdx.x, gridDim.x);
+}
+
 // Sum the row-block partials of each segment: 64 channels x 16 item lanes
 // per block, combined in LDS.
-__global__ __launch_bounds__(1024) void k_seg_reduce_u8(const uint2 *__restrict__ part,
-                                                        const int *__restrict__ seg_items,
-                                                        int nseg, int Csub,
-                                                        Part *__restrict__ segsum) {
+__device__ __forceinline__ void reduce_u8_body(const ReduceArgs &ra, uint32_t bid, uint32_t) {
+  const uint2 *__restrict__ part = (const uint2 *)ra.part;
+  const int *__restrict__ seg_items = ra.seg_items;
+  const int Csub = ra.Csub;
+  Part *__restrict__ segsum = ra.segsum;
   __shared__ uint64_t la[16][64], lb[16][64];
   const int nchunk = (Csub + 63) / 64;
-  const int s = blockIdx.x / nchunk;
-  const int c = (blockIdx.x % nchunk) * 64 + (threadIdx.x & 63);
+  const int s = (int)bid / nchunk;
+  const int c = ((int)bid % nchunk) * 64 + (threadIdx.x & 63);
   const int il = threadIdx.x >> 6;
   uint64_t a = 0, b = 0;
   if (c < Csub)
@@ -153,6 +159,10 @@ __global__ __launch_bounds__(1024) void k_seg_reduce_u8(const uint2 *__restrict_
     }
     segsum[(size_t)s * Csub + c] = Part{a, b, 0};
   }
+}
+
+__global__ __launch_bounds__(1024) void k_seg_reduce_u8(ReduceArgs a) {
+  reduce_u8_body(a, blockIdx.x, gridDim.x);
 }
 
 // ------------------------------------------------ clean + transpose (u8) --
@@ -180,11 +190,10 @@ __device__ __forceinline__ uint32_t nonzero_bytes(uint32_t w) {
   return __builtin_popcount(t & 0x80808080u);
 }
 
-__global__ __launch_bounds__(1024) void k_clean_u8(CleanArgs a) {
+__device__ __forceinline__ void clean_u8_body(const CleanArgs &a, uint32_t bid, uint32_t) {
   __shared__ uint32_t s_sum[32], s_cnt[32];
   __shared__ float s_m[32];
   if (a.run_if && *a.run_if == 0) return;
-  const uint32_t bid = xcd_remap_u8(blockIdx.x, gridDim.x);
   const RowItem it = a.items[bid];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int q = lane & 3;
@@ -318,18 +327,21 @@ __global__ __launch_bounds__(1024) void k_clean_u8(CleanArgs a) {
   }
 }
 
+__global__ __launch_bounds__(1024) void k_clean_u8(CleanArgs a) {
+  clean_u8_body(a, xcd_remap_u8(blockIdx.x, gridDim.x), gridDim.x);
+}
+
 // --------------------------------------------- subband stage 2 (u16 in) --
 // One workgroup (512 threads) per (coarse DM group, tile of 64*P outputs):
 // the group's partial sums for the tile plus its largest residual are staged
 // in LDS with 16-byte loads; each wave then sums the subbands of one fine DM
 // at a time, each lane P consecutive outputs read as P/8 16-byte LDS loads.
 template <int P>
-__global__ __launch_bounds__(512) void k_stage2_u16v(Stage2Args a) {
+__device__ __forceinline__ void stage2_u16v_body(const Stage2Args &a, uint32_t lid, uint32_t) {
   extern __shared__ uint4 s2v[];
   uint16_t *s2 = (uint16_t *)s2v;
   constexpr int TILE = 64 * P;
   const int ntile = (int)((a.nout + TILE - 1) / TILE);
-  const uint32_t lid = xcd_remap_u8(blockIdx.x, gridDim.x);
   const int ti = (int)(lid % (uint32_t)ntile);
   const int j = (int)(lid / (uint32_t)ntile);
   const size_t t0 = (size_t)ti * TILE;
@@ -363,7 +375,7 @@ __global__ __launch_bounds__(512) void k_stage2_u16v(Stage2Args a) {
         acc[8 * h + 6] += v.w & 0xFFFFu; acc[8 * h + 7] += v.w >> 16;
       }
     }
-    uint32_t *o = (uint32_t *)a.out + (size_t)m * a.nout + t0 + (size_t)lane * P;
+    uint32_t *o = (uint32_t *)a.out + (size_t)m * a.ostride + t0 + (size_t)lane * P;
     if (t0 + (size_t)lane * P + P <= a.nout) {
       __builtin_memcpy(o, acc, 4 * P);
     } else {
@@ -372,6 +384,38 @@ __global__ __launch_bounds__(512) void k_stage2_u16v(Stage2Args a) {
     }
   }
 }
+
+template <int P>
+__global__ __launch_bounds__(512) void k_stage2_u16v(Stage2Args a) {
+  stage2_u16v_body<P>(a, xcd_remap_u8(blockIdx.x, gridDim.x), gridDim.x);
+}
+
+// ------------------------------------------------------ batched launches --
+__device__ __forceinline__ int batch_find_u8(const uint32_t *__restrict__ start, int n, uint32_t g) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+#define AF_BATCHED_U8(NAME, ARGS, BOUNDS, CALL)                                         \
+  __global__ __launch_bounds__(BOUNDS) void NAME(const ARGS *__restrict__ tab,          \
+                                                 const uint32_t *__restrict__ start, int n) { \
+    const uint32_t g = xcd_remap_u8(blockIdx.x, gridDim.x);                             \
+    const int i = batch_find_u8(start, n, g);                                           \
+    const ARGS a = tab[i];                                                              \
+    const uint32_t lid = g - start[i], nb = start[i + 1] - start[i];                    \
+    CALL;                                                                               \
+  }
+
+AF_BATCHED_U8(kb_stat_u8, StatArgs, 1024, (stats_u8_body(a, lid, nb)))
+AF_BATCHED_U8(kb_red_u8, ReduceArgs, 1024, (reduce_u8_body(a, lid, nb)))
+AF_BATCHED_U8(kb_clean_u8, CleanArgs, 1024, (clean_u8_body(a, lid, nb)))
+AF_BATCHED_U8(kb_s2_v8, Stage2Args, 512, (stage2_u16v_body<8>(a, lid, nb)))
+AF_BATCHED_U8(kb_s2_v16, Stage2Args, 512, (stage2_u16v_body<16>(a, lid, nb)))
 
 // ------------------------------------------------ served-beam helpers --
 // Window statistics of a served beam from its ring of per-chunk sums: one
@@ -479,39 +523,81 @@ hipError_t launch_seg_stats_u8(const uint8_t *x, int nchan, int c0, int c1, cons
   // one block per unit (the persistent form measured slower: its last
   // round leaves CUs idle while a full grid overlaps block tails)
   const int nunits = nitems * nchunk;
-  const int grid = nunits;
-  hipLaunchKernelGGL(k_seg_stats_u8, dim3((unsigned)grid), dim3(kStatThreads), 0, st,
-                     x, nchan, c0, c1, items, part, nunits);
-  return hipGetLastError();
+  StatArgs a{x, items, part, nchan, c0, c1, nunits};
+  return emit_args(KB_STAT_U8, 1, (size_t)nunits, kStatThreads, 0, a, st);
 }
 
 hipError_t launch_seg_reduce_u8(const uint2 *part, const int *seg_items, int nseg, int Csub,
                                 Part *segsum, hipStream_t st) {
   const int nchunk = (Csub + 63) / 64;
-  hipLaunchKernelGGL(k_seg_reduce_u8, dim3((unsigned)(nseg * nchunk)), dim3(1024), 0, st, part,
-                     seg_items, nseg, Csub, segsum);
-  return hipGetLastError();
+  ReduceArgs a{part, seg_items, segsum, nseg, Csub};
+  return emit_args(KB_RED_U8, 2, (size_t)nseg * nchunk, 1024, 0, a, st);
 }
 
 hipError_t launch_clean_u8(const CleanArgs &a, hipStream_t st) {
-  hipLaunchKernelGGL(k_clean_u8, dim3((unsigned)a.nitems), dim3(1024), 0, st, a);
-  return hipGetLastError();
+  return emit_args(KB_CLEAN_U8, 6, (size_t)a.nitems, 1024, 0, a, st);
 }
 
 hipError_t launch_stage2_u16v(const Stage2Args &a, hipStream_t st) {
   const int P = a.tile / 64;
+  if (P != 8 && P != 16) return hipErrorInvalidValue;
   const int ntile = (int)((a.nout + a.tile - 1) / a.tile);
   const size_t nblk = (size_t)ntile * a.ndm_nom;
   const size_t lds = (size_t)a.nsb * a.lds_len * sizeof(uint16_t);
-  if (P == 16) {
-    (void)hipFuncSetAttribute((const void *)k_stage2_u16v<16>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_stage2_u16v<16>, dim3((unsigned)nblk), dim3(512), lds, st, a);
-  } else if (P == 8) {
+  return emit_args(P == 16 ? KB_S2_V16 : KB_S2_V8, 8, nblk, 512, lds, a, st);
+}
+
+template <typename A>
+static const A &argsof_u8(const LaunchDesc &d) {
+  return *reinterpret_cast<const A *>(d.args);
+}
+
+hipError_t launch_desc_u8(const LaunchDesc &d, hipStream_t st) {
+  const dim3 g(d.nblk), b(d.threads);
+  switch (d.kind) {
+  case KB_STAT_U8: hipLaunchKernelGGL(k_seg_stats_u8, g, b, d.lds, st, argsof_u8<StatArgs>(d)); break;
+  case KB_RED_U8: hipLaunchKernelGGL(k_seg_reduce_u8, g, b, d.lds, st, argsof_u8<ReduceArgs>(d)); break;
+  case KB_CLEAN_U8: hipLaunchKernelGGL(k_clean_u8, g, b, d.lds, st, argsof_u8<CleanArgs>(d)); break;
+  case KB_S2_V8:
     (void)hipFuncSetAttribute((const void *)k_stage2_u16v<8>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_stage2_u16v<8>, dim3((unsigned)nblk), dim3(512), lds, st, a);
-  } else {
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)d.lds);
+    hipLaunchKernelGGL(k_stage2_u16v<8>, g, b, d.lds, st, argsof_u8<Stage2Args>(d));
+    break;
+  case KB_S2_V16:
+    (void)hipFuncSetAttribute((const void *)k_stage2_u16v<16>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)d.lds);
+    hipLaunchKernelGGL(k_stage2_u16v<16>, g, b, d.lds, st, argsof_u8<Stage2Args>(d));
+    break;
+  default:
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batched_u8(int kind, const void *tab, const uint32_t *start, int n,
+                             uint32_t total, uint32_t lds, hipStream_t st) {
+  const dim3 g(total);
+  switch (kind) {
+  case KB_STAT_U8:
+    hipLaunchKernelGGL(kb_stat_u8, g, dim3(1024), lds, st, (const StatArgs *)tab, start, n);
+    break;
+  case KB_RED_U8:
+    hipLaunchKernelGGL(kb_red_u8, g, dim3(1024), lds, st, (const ReduceArgs *)tab, start, n);
+    break;
+  case KB_CLEAN_U8:
+    hipLaunchKernelGGL(kb_clean_u8, g, dim3(1024), lds, st, (const CleanArgs *)tab, start, n);
+    break;
+  case KB_S2_V8:
+    (void)hipFuncSetAttribute((const void *)kb_s2_v8, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    hipLaunchKernelGGL(kb_s2_v8, g, dim3(512), lds, st, (const Stage2Args *)tab, start, n);
+    break;
+  case KB_S2_V16:
+    (void)hipFuncSetAttribute((const void *)kb_s2_v16, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    hipLaunchKernelGGL(kb_s2_v16, g, dim3(512), lds, st, (const Stage2Args *)tab, start, n);
+    break;
+  default:
     return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -243,6 +243,12 @@ static af_status build(af_plan *p) {
     int tile = 0;
     if (inter16) {
       auto lds_of = [&](int t) { return (size_t)p->nsb * (size_t)((t + p->maxres + 7) / 8 * 8) * 2; };
+      // a requested stage-2 tile (autotuner), when it fits
+      if ((d.tile_t == 1024 || d.tile_t == 512) && lds_of(d.tile_t) <= (size_t)maxlds)
+        tile = d.tile_t;
+      if ((d.tile_t == 256 || d.tile_t == 128 || d.tile_t == 64) &&
+          lds_of(d.tile_t) <= (size_t)std::min(maxlds, 65536))
+        tile = d.tile_t;
       // 16-byte-load kernel: prefer two workgroups per CU, then one
       const int big[] = {1024, 512};
       for (int half = 2; half >= 1 && !tile; --half)
@@ -258,7 +264,19 @@ static af_status build(af_plan *p) {
     if (tile == 0) tile = 512;  // generic stage 2 (no LDS)
     p->s2_tile = tile;
     p->s2_lds = (tile + p->maxres + 7) / 8 * 8;
-    size_t ntile2 = (p->nout + tile - 1) / tile;
+    // Time super-tiles: the subband intermediate holds one super-tile of T2
+    // outputs (plus the residual halo), at most kInterBudget bytes, as the
+    // reference tiles time (src/core/dedispered.cu:37-39,1199-1235).
+    p->inter_elem = inter16 ? 2 : (d.dtype == AF_U32 || d.dtype == AF_F32 ? 8 : 4);
+    const size_t per_t = (size_t)p->ndm_nom * p->nsb * (size_t)p->inter_elem;
+    const size_t budget = d.inter_budget ? d.inter_budget : kInterBudget;
+    size_t tmax = budget / std::max<size_t>(per_t, 1);
+    tmax = tmax > (size_t)p->maxres + 2048 ? tmax - (size_t)p->maxres - 1024 : 1024;
+    tmax = std::max<size_t>(1024, tmax / 1024 * 1024);
+    const size_t nout_r = (p->nout + 1023) / 1024 * 1024;
+    p->T2 = std::min(nout_r, tmax);
+    p->nsuper = (int)((p->nout + p->T2 - 1) / p->T2);
+    size_t ntile2 = (p->T2 + tile - 1) / tile;
     n_ext = ntile2 * (size_t)tile + (size_t)p->maxres + 8;
     p->adds = (double)p->ndm_nom * p->nch * (double)(p->nout + p->maxres) +
               (double)p->ndm * p->nsb * (double)p->nout;
@@ -285,7 +303,8 @@ static af_status build(af_plan *p) {
   // Row length of the channel-major buffer: every load of the dedispersion
   // kernels (tile start + delay + 16 bytes, or + the LDS window) stays inside
   // its own channel row, which is zero beyond nbin.
-  size_t need_len = p->L1 + (size_t)p->maxdelay + 64 + (size_t)p->dd_lds_win;
+  size_t need_len = (size_t)(p->nsuper - 1) * p->T2 + p->L1 + (size_t)p->maxdelay + 64 +
+                    (size_t)p->dd_lds_win;
   p->Lp = std::max(p->nbin, need_len);
   p->Lp = (p->Lp + 255) / 256 * 256;
 
@@ -435,7 +454,6 @@ static af_status build(af_plan *p) {
   AF_TRY(dalloc(p, &p->d_xT, (size_t)p->nch * p->Lp * es));
   AF_HIP(hipMemset(p->d_xT, 0, (size_t)p->nch * p->Lp * es));
   if (d.method == AF_METHOD_SUBBAND) {
-    p->inter_elem = (d.dtype == AF_U8 && d.sb_nchan <= 256) ? 2 : (d.dtype == AF_U32 || d.dtype == AF_F32 ? 8 : 4);
     AF_TRY(dalloc(p, &p->d_inter, (size_t)p->ndm_nom * p->nsb * p->L1 * p->inter_elem));
   }
   if (d.use_iqrm && p->nwin > 0) {
@@ -519,8 +537,7 @@ af_status execute(af_plan *p, const void *d_in, void *d_out, uint8_t *d_masks, h
   AF_HIP(launch_clean(d.dtype, ca, st));
   AF_TRY(record(p, set, 4, st));
   AF_TRY(run_dedisp(p, p->d_xT, p->Lp, d_out, st, set));
-  if (d_masks && iq)
-    AF_HIP(hipMemcpyAsync(d_masks, p->d_masks, (size_t)p->nwin * d.nchan, hipMemcpyDeviceToDevice, st));
+  if (d_masks && iq) AF_HIP(launch_copy(d_masks, p->d_masks, (size_t)p->nwin * d.nchan, st));
   return AF_OK;
 }
 
@@ -563,6 +580,7 @@ af_status run_dedisp(af_plan *p, const void *xT, size_t Lp, void *d_out, hipStre
     AF_TRY(record(p, set, 5, st));
   } else {
     const bool inter16 = d.dtype == AF_U8 && d.sb_nchan <= 256;
+    const size_t es = dtype_size(d.dtype), os = (size_t)acc_size(d.dtype);
     da.nrows = p->ndm_nom;
     da.ngrp = p->nsb;
     da.grp_w = d.sb_nchan;
@@ -570,8 +588,6 @@ af_status run_dedisp(af_plan *p, const void *xT, size_t Lp, void *d_out, hipStre
     da.ostride = p->L1;
     da.nvalid = p->L1;
     da.out16 = inter16 ? 1 : 0;
-    AF_HIP(launch_dedisperse(d.dtype, da, st));
-    AF_TRY(record(p, set, 5, st));
     Stage2Args sa{};
     sa.inter = p->d_inter;
     sa.L1 = p->L1;
@@ -581,12 +597,21 @@ af_status run_dedisp(af_plan *p, const void *xT, size_t Lp, void *d_out, hipStre
     sa.ndm = p->ndm;
     sa.ndm0 = d.ndm0;
     sa.ndm_nom = p->ndm_nom;
-    sa.out = d_out;
-    sa.nout = p->nout;
+    sa.ostride = p->nout;
     sa.tile = p->s2_tile;
     sa.lds_len = p->s2_lds;
     sa.inter16 = inter16 ? 1 : 0;
-    AF_HIP(launch_stage2(d.dtype, sa, st));
+    // super-tile k: stage 1 reads the cleaned buffer from column k*T2,
+    // stage 2 writes output columns [k*T2, min(nout, (k+1)*T2))
+    for (int k = 0; k < p->nsuper; ++k) {
+      const size_t t0 = (size_t)k * p->T2;
+      da.xT = (const uint8_t *)xT + t0 * es;
+      AF_HIP(launch_dedisperse(d.dtype, da, st));
+      if (k + 1 == p->nsuper) AF_TRY(record(p, set, 5, st));
+      sa.out = (uint8_t *)d_out + t0 * os;
+      sa.nout = std::min(p->T2, p->nout - t0);
+      AF_HIP(launch_stage2(d.dtype, sa, st));
+    }
   }
   AF_TRY(record(p, set, 6, st));
   return AF_OK;

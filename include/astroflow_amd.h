@@ -80,8 +80,10 @@ typedef struct af_plan_desc {
   const int *bad_chans;
   int n_bad;
   /* kernel tuning; 0 = library default (the result does not depend on these) */
-  int tile_t;              /* output samples per workgroup tile                 */
-  int dm_per_block;        /* DM rows per workgroup                             */
+  int tile_t;              /* stage-2 output tile: 64..1024 (0 = chosen by LDS)  */
+  int dm_per_block;        /* DM rows per stage-1 workgroup: 4 or 16 (0 = 4)    */
+  size_t inter_budget;     /* bytes of subband intermediate per time super-tile
+                              (0 = 1 GiB); longer outputs run in super-tiles  */
 } af_plan_desc;
 
 typedef struct af_plan af_plan;
@@ -164,6 +166,39 @@ af_status af_sync(void);
  * stream, bracketed by HIP events on that stream. */
 af_status af_time_execute(af_plan *plan, const void *d_in, void *d_out, int reps,
                           float *ms);
+
+/* --------------------------------------------------------------- batch --
+ * Many plans (beams, chunks of different nchan, nsamp, DM range, dtype)
+ * executed together.  Replaces a loop of per-chunk calls of
+ * gpucal::dedisperse_spec<T> (src/core/dedispered.cu:988-1463), each stage a
+ * separate launch, with one launch per stage and kernel for the whole batch:
+ * the launches every plan's af_execute would make are merged, each
+ * workgroup finding its plan in a table.  Results equal af_execute of each
+ * plan on its buffers.  The buffers are bound at creation; plans and
+ * buffers must outlive the batch, and a plan must not run elsewhere while
+ * the batch runs (they share the plan's scratch). */
+typedef struct af_batch af_batch;
+
+/* d_masks may be NULL (or hold NULL entries). */
+af_status af_batch_create(af_plan *const *plans, int nplans, const void *const *d_ins,
+                          void *const *d_outs, uint8_t *const *d_masks, af_batch **batch);
+/* use_graph != 0: the launch sequence is captured into a hipGraph on the
+ * first call and replayed after.  Asynchronous on the stream. */
+af_status af_batch_execute(af_batch *batch, int use_graph, void *stream);
+/* Launches per execute of the batch, and of the plans run one by one. */
+af_status af_batch_info(const af_batch *batch, int *launches, int *recorded);
+af_status af_batch_destroy(af_batch *batch);
+
+/* ----------------------------------------------------------- autotuner --
+ * Times the result-invariant kernel choices of a plan on the caller's
+ * buffers -- stage-1 DM rows per workgroup {4, 16} and stage-2 output tile
+ * {256, 512, 1024} -- and, when tune_subbands != 0, the subband width (16,
+ * 32, 64, 128 channels: this changes the method's rounding of delays and
+ * therefore the plane).  Writes the fastest settings into *desc (tile_t,
+ * dm_per_block, sb_nchan) and returns the best time per execute.  Results
+ * are kept per shape for the life of the process. */
+af_status af_autotune(af_plan_desc *desc, const void *d_in, void *d_out, int tune_subbands,
+                      int reps, float *best_ms);
 
 /* Page-locked host memory (hipHostMalloc), for asynchronous uploads. */
 af_status af_host_alloc(void **p, size_t bytes);

@@ -153,3 +153,25 @@ def test_delay_tables_equal_oracle(gpu_lib):
     assert np.array_equal(res, O.residual_table(f, 0, 4096, 128, f[-1], float(b.tsamp), 1,
                                                 0.0, 0.5, 6001, 32))
     plan.close()
+
+
+@pytest.mark.parametrize("dtype,tile", [(np.uint8, 0), (np.uint8, 256), (np.uint16, 0),
+                                        (np.float32, 0)])
+def test_subband_time_super_tiles(gpu_lib, dtype, tile):
+    # a small intermediate budget splits the output into several time
+    # super-tiles (stage 1 + stage 2 per super-tile); the plane is unchanged
+    say(f"case: subband {np.dtype(dtype).name} in time super-tiles (stage-2 tile {tile})")
+    rng = np.random.default_rng(21)
+    b = small_band(384, 1100.0, 1500.0, 1e-4)
+    x = noise(rng, 20000, 384, dtype)
+    spec = make_spec(x, b, 0.0, 1.0, 150, method="subband", sbw=64, ndm0=16, nout=17000,
+                     use_zero_dm=True, zdm_c0=0, zdm_c1=384, tile_t=tile,
+                     inter_budget=10 * 6 * 2 * 4096)
+    plan = capi.Plan(spec)
+    assert plan.info.inter_bytes <= 3 * 10 * 6 * 2 * 4096 + (1 << 20)
+    plan.close()
+    got, ref, *_ = run_both(x, spec, want_masks=False)
+    if dtype == np.float32:
+        assert_f32_close(got, ref, x, spec)
+    else:
+        assert np.array_equal(got, ref)
