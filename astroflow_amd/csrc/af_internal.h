@@ -135,6 +135,7 @@ struct Stage2Args {
   int tile;                 // outputs per workgroup tile
   int lds_len;              // LDS row length (tile + max over groups of maxres)
   int inter16;              // inter is u16 (u8 input)
+  int sbc;                  // u16 LDS kernel: subbands staged at a time (0 = all)
 };
 hipError_t launch_stage2(int dtype, const Stage2Args &a, hipStream_t st);
 // Device-to-device copy on `st` (recorded like the kernels, see below).
@@ -197,7 +198,7 @@ enum LaunchKind : int {
   KB_CLEAN_U8, KB_CLEAN_U16, KB_CLEAN_U32, KB_CLEAN_F32,
   KB_DD_U8Q, KB_DD_U8, KB_DD_U8LDS, KB_DD_G_U8, KB_DD_G_U16, KB_DD_G_U32, KB_DD_G_F32F,
   KB_DD_G_F32D,
-  KB_S2_V8, KB_S2_V16, KB_S2_U16, KB_S2_G_U32, KB_S2_G_U64, KB_S2_G_F32,
+  KB_S2_V8, KB_S2_V16, KB_S2_C8, KB_S2_C16, KB_S2_U16, KB_S2_G_U16, KB_S2_G_U32, KB_S2_G_U64, KB_S2_G_F32,
   KB_COPY,
   KB_COUNT
 };
@@ -257,7 +258,7 @@ struct af_plan {
   size_t T2 = 0;            // outputs per time super-tile (subband method)
   int nsuper = 1;           // super-tiles
   int inter_elem = 0;       // bytes per subband partial sum
-  int s2_tile = 0, s2_lds = 0, maxlds = 0;
+  int s2_tile = 0, s2_lds = 0, s2_sbc = 0, maxlds = 0;
   int dd_lds_rows = 0, dd_lds_win = 0;   // LDS-staged u8 dedispersion
   std::vector<int4> worklist;            // stage-1 workgroups in locality order
   std::vector<int2> chan_order;          // (channel, delay) sorted by delay quarter class

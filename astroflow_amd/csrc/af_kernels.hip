@@ -897,23 +897,24 @@ __global__ __launch_bounds__(256) void k_stage2_u16(Stage2Args a) {
   stage2_u16_body(a, xcd_remap(blockIdx.x, gridDim.x), gridDim.x);
 }
 
-// Stage 2 for u32/u64/double partial sums, straight from HBM/L2.
-template <typename Acc, typename OutT>
+// Stage 2 from partial sums of type In (u16/u32/u64/double), accumulated
+// in Acc, straight from HBM/L2.
+template <typename In, typename Acc, typename OutT>
 __device__ __forceinline__ void stage2_generic_body(const Stage2Args &a, uint32_t bid, uint32_t) {
   const size_t idx = (size_t)bid * blockDim.x + threadIdx.x;
   if (idx >= (size_t)a.ndm * a.nout) return;
   const int m = (int)(idx / a.nout);
   const size_t t = idx % a.nout;
   const int j = m / a.ndm0;
-  const Acc *src = (const Acc *)a.inter + (size_t)j * a.nsb * a.L1 + t;
+  const In *src = (const In *)a.inter + (size_t)j * a.nsb * a.L1 + t;
   Acc acc = 0;
-  for (int sb = 0; sb < a.nsb; ++sb) acc += src[(size_t)sb * a.L1 + a.res[(size_t)m * a.nsb + sb]];
+  for (int sb = 0; sb < a.nsb; ++sb) acc += (Acc)src[(size_t)sb * a.L1 + a.res[(size_t)m * a.nsb + sb]];
   ((OutT *)a.out)[(size_t)m * a.ostride + t] = (OutT)acc;
 }
 
-template <typename Acc, typename OutT>
+template <typename In, typename Acc, typename OutT>
 __global__ __launch_bounds__(256) void k_stage2_generic(Stage2Args a) {
-  stage2_generic_body<Acc, OutT>(a, blockIdx.x, gridDim.x);
+  stage2_generic_body<In, Acc, OutT>(a, blockIdx.x, gridDim.x);
 }
 
 // ------------------------------------------------------ batched launches --
@@ -968,9 +969,10 @@ AF_BATCHED(kb_dd_g_u32, DdArgs, 256, (dd_generic_body<uint32_t, uint64_t, uint64
 AF_BATCHED(kb_dd_g_f32f, DdArgs, 256, (dd_generic_body<float, double, float>(a, lid, nb)))
 AF_BATCHED(kb_dd_g_f32d, DdArgs, 256, (dd_generic_body<float, double, double>(a, lid, nb)))
 AF_BATCHED(kb_s2_u16, Stage2Args, 256, (stage2_u16_body(a, lid, nb)))
-AF_BATCHED(kb_s2_g_u32, Stage2Args, 256, (stage2_generic_body<uint32_t, uint32_t>(a, lid, nb)))
-AF_BATCHED(kb_s2_g_u64, Stage2Args, 256, (stage2_generic_body<uint64_t, uint64_t>(a, lid, nb)))
-AF_BATCHED(kb_s2_g_f32, Stage2Args, 256, (stage2_generic_body<double, float>(a, lid, nb)))
+AF_BATCHED(kb_s2_g_u16, Stage2Args, 256, (stage2_generic_body<uint16_t, uint32_t, uint32_t>(a, lid, nb)))
+AF_BATCHED(kb_s2_g_u32, Stage2Args, 256, (stage2_generic_body<uint32_t, uint32_t, uint32_t>(a, lid, nb)))
+AF_BATCHED(kb_s2_g_u64, Stage2Args, 256, (stage2_generic_body<uint64_t, uint64_t, uint64_t>(a, lid, nb)))
+AF_BATCHED(kb_s2_g_f32, Stage2Args, 256, (stage2_generic_body<double, double, float>(a, lid, nb)))
 
 // ---------------------------------------------------------- launchers --
 
@@ -1076,19 +1078,16 @@ hipError_t launch_dedisperse(int dtype, const DdArgs &a, hipStream_t st) {
 }
 
 hipError_t launch_stage2(int dtype, const Stage2Args &a, hipStream_t st) {
-  if (dtype == AF_U8 && a.inter16 && (a.tile == 512 || a.tile == 1024)) {
-    const int ntile = (int)((a.nout + a.tile - 1) / a.tile);
-    const size_t lds = (size_t)a.nsb * a.lds_len * sizeof(uint16_t);
-    return emit(a.tile == 1024 ? KB_S2_V16 : KB_S2_V8, 8, (size_t)ntile * a.ndm_nom, 512, lds, a,
-                st);
-  }
-  if (dtype == AF_U8 && a.inter16) {
+  if (dtype == AF_U8 && a.inter16 && (a.tile == 512 || a.tile == 1024))
+    return launch_stage2_u16v(a, st);   // all subbands in LDS, or in chunks
+  if (dtype == AF_U8 && a.inter16 && a.tile > 0) {
     const int ntile = (int)((a.nout + a.tile - 1) / a.tile);
     const size_t lds = (size_t)a.nsb * a.lds_len * sizeof(uint16_t);
     return emit(KB_S2_U16, 8, (size_t)ntile * a.ndm_nom, 256, lds, a, st);
   }
   const size_t n = (size_t)a.ndm * a.nout;
   const size_t g = (n + 255) / 256;
+  if (dtype == AF_U8 && a.inter16) return emit(KB_S2_G_U16, 8, g, 256, 0, a, st);
   switch (dtype) {
   case AF_U8:
   case AF_U16: return emit(KB_S2_G_U32, 8, g, 256, 0, a, st);
@@ -1155,14 +1154,21 @@ hipError_t launch_desc(const LaunchDesc &d, hipStream_t st) {
     hipLaunchKernelGGL((k_dd_generic<float, double, double>), g, b, d.lds, st, argsof<DdArgs>(d));
     break;
   case KB_S2_U16: hipLaunchKernelGGL(k_stage2_u16, g, b, d.lds, st, argsof<Stage2Args>(d)); break;
+  case KB_S2_G_U16:
+    hipLaunchKernelGGL((k_stage2_generic<uint16_t, uint32_t, uint32_t>), g, b, d.lds, st,
+                       argsof<Stage2Args>(d));
+    break;
   case KB_S2_G_U32:
-    hipLaunchKernelGGL((k_stage2_generic<uint32_t, uint32_t>), g, b, d.lds, st, argsof<Stage2Args>(d));
+    hipLaunchKernelGGL((k_stage2_generic<uint32_t, uint32_t, uint32_t>), g, b, d.lds, st,
+                       argsof<Stage2Args>(d));
     break;
   case KB_S2_G_U64:
-    hipLaunchKernelGGL((k_stage2_generic<uint64_t, uint64_t>), g, b, d.lds, st, argsof<Stage2Args>(d));
+    hipLaunchKernelGGL((k_stage2_generic<uint64_t, uint64_t, uint64_t>), g, b, d.lds, st,
+                       argsof<Stage2Args>(d));
     break;
   case KB_S2_G_F32:
-    hipLaunchKernelGGL((k_stage2_generic<double, float>), g, b, d.lds, st, argsof<Stage2Args>(d));
+    hipLaunchKernelGGL((k_stage2_generic<double, double, float>), g, b, d.lds, st,
+                       argsof<Stage2Args>(d));
     break;
   case KB_COPY: {
     const CopyArgs &c = argsof<CopyArgs>(d);
@@ -1213,6 +1219,7 @@ hipError_t launch_batched(int kind, const void *tab, const uint32_t *start, int 
     AF_BL(KB_DD_G_F32F, kb_dd_g_f32f, 256, DdArgs)
     AF_BL(KB_DD_G_F32D, kb_dd_g_f32d, 256, DdArgs)
     AF_BL(KB_S2_U16, kb_s2_u16, 256, Stage2Args)
+    AF_BL(KB_S2_G_U16, kb_s2_g_u16, 256, Stage2Args)
     AF_BL(KB_S2_G_U32, kb_s2_g_u32, 256, Stage2Args)
     AF_BL(KB_S2_G_U64, kb_s2_g_u64, 256, Stage2Args)
     AF_BL(KB_S2_G_F32, kb_s2_g_f32, 256, Stage2Args)

@@ -390,6 +390,80 @@ __global__ __launch_bounds__(512) void k_stage2_u16v(Stage2Args a) {
   stage2_u16v_body<P>(a, xcd_remap_u8(blockIdx.x, gridDim.x), gridDim.x);
 }
 
+// Stage 2 when all subbands' tile + halo do not fit LDS (many subbands, a
+// large residual halo): the workgroup stages a.sbc subbands at a time, and
+// each wave keeps the sums of its (up to 4) fine DMs in registers across the
+// chunks.  Requires ndm0 <= 32 (8 waves x 4 DMs).
+template <int P>
+__device__ __forceinline__ void stage2_u16c_body(const Stage2Args &a, uint32_t lid, uint32_t) {
+  extern __shared__ uint4 s2v[];
+  uint16_t *s2 = (uint16_t *)s2v;
+  constexpr int TILE = 64 * P;
+  constexpr int MD = 4;
+  const int ntile = (int)((a.nout + TILE - 1) / TILE);
+  const int ti = (int)(lid % (uint32_t)ntile);
+  const int j = (int)(lid / (uint32_t)ntile);
+  const size_t t0 = (size_t)ti * TILE;
+  const int ext8 = (TILE + a.maxres_j[j] + 7) / 8;
+  const int lds8 = a.lds_len / 8;
+  const uint16_t *__restrict__ src = (const uint16_t *)a.inter + (size_t)j * a.nsb * a.L1 + t0;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int nw = blockDim.x >> 6;
+  uint32_t acc[MD][P];
+#pragma unroll
+  for (int d = 0; d < MD; ++d)
+#pragma unroll
+    for (int i = 0; i < P; ++i) acc[d][i] = 0;
+  for (int sb0 = 0; sb0 < a.nsb; sb0 += a.sbc) {
+    const int nsc = min(a.sbc, a.nsb - sb0);
+    for (int idx = threadIdx.x; idx < nsc * ext8; idx += blockDim.x) {
+      const int sb = idx / ext8, k = idx % ext8;
+      s2v[sb * lds8 + k] = *(const uint4 *)(src + (size_t)(sb0 + sb) * a.L1 + 8 * k);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < MD; ++d) {
+      const int mi = wave + d * nw;
+      const int m = j * a.ndm0 + mi;
+      if (mi < a.ndm0 && m < a.ndm) {
+        const int32_t *__restrict__ rrow = a.res + (size_t)m * a.nsb + sb0;
+        for (int sb = 0; sb < nsc; ++sb) {
+          const uint16_t *p = s2 + sb * a.lds_len + lane * P + rrow[sb];
+#pragma unroll
+          for (int h = 0; h < P / 8; ++h) {
+            uint4 v;
+            __builtin_memcpy(&v, p + 8 * h, 16);
+            acc[d][8 * h + 0] += v.x & 0xFFFFu; acc[d][8 * h + 1] += v.x >> 16;
+            acc[d][8 * h + 2] += v.y & 0xFFFFu; acc[d][8 * h + 3] += v.y >> 16;
+            acc[d][8 * h + 4] += v.z & 0xFFFFu; acc[d][8 * h + 5] += v.z >> 16;
+            acc[d][8 * h + 6] += v.w & 0xFFFFu; acc[d][8 * h + 7] += v.w >> 16;
+          }
+        }
+      }
+    }
+    __syncthreads();   // the next chunk overwrites LDS
+  }
+#pragma unroll
+  for (int d = 0; d < MD; ++d) {
+    const int mi = wave + d * nw;
+    const int m = j * a.ndm0 + mi;
+    if (mi >= a.ndm0 || m >= a.ndm) continue;
+    uint32_t *o = (uint32_t *)a.out + (size_t)m * a.ostride + t0 + (size_t)lane * P;
+    if (t0 + (size_t)lane * P + P <= a.nout) {
+      __builtin_memcpy(o, acc[d], 4 * P);
+    } else {
+      for (int i = 0; i < P; ++i)
+        if (t0 + (size_t)lane * P + i < a.nout) o[i] = acc[d][i];
+    }
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(512) void k_stage2_u16c(Stage2Args a) {
+  stage2_u16c_body<P>(a, xcd_remap_u8(blockIdx.x, gridDim.x), gridDim.x);
+}
+
 // ------------------------------------------------------ batched launches --
 __device__ __forceinline__ int batch_find_u8(const uint32_t *__restrict__ start, int n, uint32_t g) {
   int lo = 0, hi = n - 1;
@@ -416,6 +490,8 @@ AF_BATCHED_U8(kb_red_u8, ReduceArgs, 1024, (reduce_u8_body(a, lid, nb)))
 AF_BATCHED_U8(kb_clean_u8, CleanArgs, 1024, (clean_u8_body(a, lid, nb)))
 AF_BATCHED_U8(kb_s2_v8, Stage2Args, 512, (stage2_u16v_body<8>(a, lid, nb)))
 AF_BATCHED_U8(kb_s2_v16, Stage2Args, 512, (stage2_u16v_body<16>(a, lid, nb)))
+AF_BATCHED_U8(kb_s2_c8, Stage2Args, 512, (stage2_u16c_body<8>(a, lid, nb)))
+AF_BATCHED_U8(kb_s2_c16, Stage2Args, 512, (stage2_u16c_body<16>(a, lid, nb)))
 
 // ------------------------------------------------ served-beam helpers --
 // Window statistics of a served beam from its ring of per-chunk sums: one
@@ -543,6 +619,11 @@ hipError_t launch_stage2_u16v(const Stage2Args &a, hipStream_t st) {
   if (P != 8 && P != 16) return hipErrorInvalidValue;
   const int ntile = (int)((a.nout + a.tile - 1) / a.tile);
   const size_t nblk = (size_t)ntile * a.ndm_nom;
+  if (a.sbc > 0 && a.sbc < a.nsb) {
+    if (a.ndm0 > 32) return hipErrorInvalidValue;
+    const size_t lds = (size_t)a.sbc * a.lds_len * sizeof(uint16_t);
+    return emit_args(P == 16 ? KB_S2_C16 : KB_S2_C8, 8, nblk, 512, lds, a, st);
+  }
   const size_t lds = (size_t)a.nsb * a.lds_len * sizeof(uint16_t);
   return emit_args(P == 16 ? KB_S2_V16 : KB_S2_V8, 8, nblk, 512, lds, a, st);
 }
@@ -567,6 +648,16 @@ hipError_t launch_desc_u8(const LaunchDesc &d, hipStream_t st) {
     (void)hipFuncSetAttribute((const void *)k_stage2_u16v<16>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)d.lds);
     hipLaunchKernelGGL(k_stage2_u16v<16>, g, b, d.lds, st, argsof_u8<Stage2Args>(d));
+    break;
+  case KB_S2_C8:
+    (void)hipFuncSetAttribute((const void *)k_stage2_u16c<8>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)d.lds);
+    hipLaunchKernelGGL(k_stage2_u16c<8>, g, b, d.lds, st, argsof_u8<Stage2Args>(d));
+    break;
+  case KB_S2_C16:
+    (void)hipFuncSetAttribute((const void *)k_stage2_u16c<16>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)d.lds);
+    hipLaunchKernelGGL(k_stage2_u16c<16>, g, b, d.lds, st, argsof_u8<Stage2Args>(d));
     break;
   default:
     return hipErrorInvalidValue;
@@ -596,6 +687,16 @@ hipError_t launch_batched_u8(int kind, const void *tab, const uint32_t *start, i
     (void)hipFuncSetAttribute((const void *)kb_s2_v16, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds);
     hipLaunchKernelGGL(kb_s2_v16, g, dim3(512), lds, st, (const Stage2Args *)tab, start, n);
+    break;
+  case KB_S2_C8:
+    (void)hipFuncSetAttribute((const void *)kb_s2_c8, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    hipLaunchKernelGGL(kb_s2_c8, g, dim3(512), lds, st, (const Stage2Args *)tab, start, n);
+    break;
+  case KB_S2_C16:
+    (void)hipFuncSetAttribute((const void *)kb_s2_c16, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    hipLaunchKernelGGL(kb_s2_c16, g, dim3(512), lds, st, (const Stage2Args *)tab, start, n);
     break;
   default:
     return hipErrorInvalidValue;

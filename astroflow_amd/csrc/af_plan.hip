@@ -254,6 +254,18 @@ static af_status build(af_plan *p) {
       for (int half = 2; half >= 1 && !tile; --half)
         for (int t : big)
           if (lds_of(t) <= (size_t)maxlds / half) { tile = t; break; }
+      // many subbands: stage them in chunks (<= 64 KB per workgroup)
+      if (!tile && d.ndm0 <= 32) {
+        for (int t : big) {
+          const size_t row = (size_t)((t + p->maxres + 7) / 8 * 8) * 2;
+          const int sbc = (int)std::min<size_t>((size_t)p->nsb, 65536 / row);
+          if (sbc >= 4 && (d.tile_t == 0 || d.tile_t == t)) {
+            tile = t;
+            p->s2_sbc = sbc;
+            break;
+          }
+        }
+      }
       // element-wise kernel within 64 KB
       if (!tile) {
         const int small[] = {256, 128, 64};
@@ -261,9 +273,11 @@ static af_status build(af_plan *p) {
           if (lds_of(t) <= (size_t)std::min(maxlds, 65536)) { tile = t; break; }
       }
     }
-    if (tile == 0) tile = 512;  // generic stage 2 (no LDS)
+    // 0: stage 2 without LDS (u16 partial sums whose halo does not fit, or
+    // wider partial sums)
     p->s2_tile = tile;
     p->s2_lds = (tile + p->maxres + 7) / 8 * 8;
+    if (tile == 0) tile = 512;   // output granularity of the super-tiles
     // Time super-tiles: the subband intermediate holds one super-tile of T2
     // outputs (plus the residual halo), at most kInterBudget bytes, as the
     // reference tiles time (src/core/dedispered.cu:37-39,1199-1235).
@@ -601,6 +615,7 @@ af_status run_dedisp(af_plan *p, const void *xT, size_t Lp, void *d_out, hipStre
     sa.tile = p->s2_tile;
     sa.lds_len = p->s2_lds;
     sa.inter16 = inter16 ? 1 : 0;
+    sa.sbc = p->s2_sbc;
     // super-tile k: stage 1 reads the cleaned buffer from column k*T2,
     // stage 2 writes output columns [k*T2, min(nout, (k+1)*T2))
     for (int k = 0; k < p->nsuper; ++k) {

@@ -149,3 +149,22 @@ def chunk_4096(seed: int = 2, threads: int = 8):
     for dm, t0 in ((150.0, 2000), (1200.0, 5000), (2800.0, 8000)):
         inject_pulse(x, f, band.f_top, tsamp, dm, t0, 1e-3, 48.0)
     return x, band, nout, dmax
+
+
+def c3_chunks(seed: int = 4, n: int = 48):
+    """48 independent beam-chunks of the batched workload: nchan in {1024,
+    2048, 4096}, nsamp log-uniform in [2^14, 2^20], dtype uint8 or float32,
+    DM upper limit in {500, 2000, 5000} with step 1 or 0.5, time downsample
+    in {1, 2, 4}.  Returns a list of dicts (sizes only; data is filled on
+    the device)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        out.append(dict(
+            nchan=int(rng.choice([1024, 2048, 4096])),
+            nsamp=int(round(2.0 ** rng.uniform(14, 20))),
+            dtype=np.uint8 if rng.random() < 0.5 else np.float32,
+            dm_max=float(rng.choice([500.0, 2000.0, 5000.0])),
+            dm_step=float(rng.choice([1.0, 0.5])),
+            ds=int(rng.choice([1, 2, 4]))))
+    return out

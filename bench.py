@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--no-served", action="store_true", help="skip the served-stream leg")
     ap.add_argument("--served-full-rfi", action="store_true",
                     help="served leg without incremental RFI (every window cleaned whole)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
+    ap.add_argument("--batch-only", action="store_true", help="run the batched leg alone")
     ap.add_argument("--served-only", action="store_true",
                     help="run the served leg alone (no plain-call line)")
     ap.add_argument("--served-chunks", type=int, default=6,
@@ -223,6 +225,135 @@ def served_leg(threads, chunks_per_beam, full_rfi=False):
     }
 
 
+def c3_spec(c):
+    """One chunk of the batched workload: 1000-1500 MHz, tsamp 49.152 us, DM
+    0..dm_max, subband (32-channel subbands, 32 fine DMs per coarse DM, the
+    reference defaults), IQRM (std, radius 5 channels, 3 sigma, whole
+    buffer) + zero-DM, full output length."""
+    from astroflow_amd import capi, synth
+    band = synth.band_1000_1500(c["nchan"])
+    f = band.freq
+    nchan = c["nchan"]
+    ndm = int(round(c["dm_max"] / c["dm_step"])) + 1
+    return capi.DedispSpec(
+        dtype=c["dtype"], nchan=nchan, nsamp=c["nsamp"], freq=f, tsamp=float(band.tsamp),
+        dm_low=0.0, dm_step=c["dm_step"], ndm=ndm, dd_c0=0, dd_c1=nchan, f_ref=float(f[-1]),
+        ds=c["ds"], nout=0, method="subband", sbw=32, ndm0=32,
+        use_iqrm=True, iqrm_c0=0, iqrm_c1=nchan - 1, iqrm_mode=1,
+        iqrm_radius_frac=float(np.float32(5.5 / (nchan - 1))), iqrm_nsigma=3.0,
+        iqrm_geofactor=1.5, iqrm_W=0, iqrm_H=0, iqrm_include_tail=True,
+        use_zero_dm=True, zdm_c0=0, zdm_c1=nchan - 1)
+
+
+def batch_leg(threads, budget_gb=110.0):
+    """The 48 seeded beam-chunks (uneven nchan, nsamp, DM range, dtype, ds),
+    each with IQRM + zero-DM: one plain execute per chunk (the reference's
+    per-call pattern, inputs resident) vs the batched launches (one launch
+    per stage and kernel) vs their hipGraph replay.  Chunks are grouped under
+    an HBM budget; every mode runs each group once after a warm-up."""
+    from astroflow_amd import capi, synth
+    chunks = synth.c3_chunks(seed=4)
+    pool_rows = 1 << 16
+    rng = np.random.default_rng(40)
+    pools = {}
+    for dt in (np.uint8, np.float32):
+        n = pool_rows * 4096
+        if dt == np.uint8:
+            a = synth.noise_u8_threaded(41, pool_rows, 4096, threads=threads)
+        else:
+            a = rng.standard_normal(n, dtype=np.float32).reshape(pool_rows, 4096)
+        d = capi.DeviceBuffer(a.nbytes)
+        d.upload(a)
+        pools[dt] = (d, a.nbytes)
+    budget = budget_gb * 1e9
+    t_plain = t_merged = t_graph = 0.0
+    work = 0
+    launches = recorded = 0
+    ngroups = 0
+    i = 0
+    t_setup = time.perf_counter()
+    while i < len(chunks):
+        group = []
+        used = 0.0
+        while i < len(chunks):
+            spec = c3_spec(chunks[i])
+            plan = capi.Plan(spec)
+            es = np.dtype(spec.dtype).itemsize
+            need = (spec.nsamp * spec.nchan * es + plan.info.scratch_bytes +
+                    plan.info.ndm * plan.info.nout * plan.info.out_elem_bytes)
+            if group and used + need > budget:
+                plan.close()
+                break
+            group.append((spec, plan))
+            used += need
+            i += 1
+        ngroups += 1
+        ins, outs = [], []
+        for spec, plan in group:
+            es = np.dtype(spec.dtype).itemsize
+            nb = spec.nsamp * spec.nchan * es
+            di = capi.DeviceBuffer(nb)
+            pool, pb = pools[spec.dtype]
+            off = 0
+            while off < nb:                      # fill from the noise pool
+                k = min(pb, nb - off)
+                capi.check(capi.lib().af_memcpy_d2d(capi.C.c_void_p(di.ptr + off),
+                                                    capi.C.c_void_p(pool.ptr), k, None))
+                off += k
+            do = capi.DeviceBuffer(plan.info.ndm * plan.info.nout * plan.info.out_elem_bytes)
+            ins.append(di)
+            outs.append(do)
+            work += plan.info.ndm * plan.info.nout * spec.ds
+        capi.sync()
+        plans = [p for _, p in group]
+        for p, di, do in zip(plans, ins, outs):            # warm-up
+            p.execute(di.ptr, do.ptr)
+        capi.sync()
+        t0 = time.perf_counter()
+        for p, di, do in zip(plans, ins, outs):
+            p.execute(di.ptr, do.ptr)
+        capi.sync()
+        t_plain += time.perf_counter() - t0
+        batch = capi.Batch(plans, [b.ptr for b in ins], [b.ptr for b in outs])
+        inf = batch.info()
+        launches += inf["launches"]
+        recorded += inf["recorded"]
+        batch.execute(use_graph=False)
+        capi.sync()
+        t0 = time.perf_counter()
+        batch.execute(use_graph=False)
+        capi.sync()
+        t_merged += time.perf_counter() - t0
+        batch.execute(use_graph=True)                     # capture + first replay
+        capi.sync()
+        t0 = time.perf_counter()
+        batch.execute(use_graph=True)
+        capi.sync()
+        t_graph += time.perf_counter() - t0
+        batch.close()
+        for p in plans:
+            p.close()
+        for b in ins + outs:
+            b.free()
+        log(f"batch group {ngroups}: {len(group)} chunks, {used / 1e9:.1f} GB; "
+            f"plain {t_plain:.3f} s merged {t_merged:.3f} s graph {t_graph:.3f} s so far")
+    for d, _ in pools.values():
+        d.free()
+    return {
+        "workload": "48 seeded beam-chunks: nchan 1024/2048/4096, nsamp log-uniform 2^14-2^20, "
+                    "u8 or f32 (N(0,1)), DM 0-500/2000/5000 step 1 or 0.5, ds 1/2/4; IQRM + "
+                    "zero-DM; 32-channel subbands; full-length planes; inputs resident in HBM",
+        "chunks": len(chunks), "groups": ngroups, "hbm_budget_GB": budget_gb,
+        "work": work, "launches_batched": launches, "launches_plain": recorded,
+        "plain_s": t_plain, "merged_s": t_merged, "graph_s": t_graph,
+        "plain_rate": work / t_plain, "merged_rate": work / t_merged,
+        "graph_rate": work / t_graph,
+        "speedup_merged_vs_plain": t_plain / t_merged,
+        "speedup_graph_vs_plain": t_plain / t_graph,
+        "setup_s": time.perf_counter() - t_setup - t_plain - t_merged - t_graph,
+    }
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -242,6 +373,9 @@ def main():
     threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)))
     if a.served_only:
         print(json.dumps({"served": served_leg(threads, a.served_chunks, a.served_full_rfi)}))
+        return
+    if a.batch_only:
+        print(json.dumps({"batched": batch_leg(threads)}))
         return
     x, spec, info = make_workload(threads)
     spec.dm_per_block = a.dm_per_block
@@ -323,6 +457,8 @@ def main():
     d_out.free()
     if not a.no_served:
         res["served"] = served_leg(threads, a.served_chunks, a.served_full_rfi)
+    if not a.no_batch:
+        res["batched"] = batch_leg(threads)
     if not a.no_cpu:
         res["cpu_baseline"] = cpu_baseline(x, info, a.cpu_seconds)
         res["speedup_vs_cpu"] = value / res["cpu_baseline"]["value"]

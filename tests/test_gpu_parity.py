@@ -175,3 +175,17 @@ def test_subband_time_super_tiles(gpu_lib, dtype, tile):
         assert_f32_close(got, ref, x, spec)
     else:
         assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("tile", [0, 512])
+def test_subband_u8_many_subbands_chunked_stage2(gpu_lib, tile):
+    # 64 subbands with a ~1400-sample residual halo do not fit LDS at once:
+    # stage 2 stages the subbands in chunks (13 at tile 1024, 16 at tile 512)
+    say(f"case: subband u8 2048ch, 32-ch subbands, large halo (stage-2 tile {tile})")
+    rng = np.random.default_rng(23)
+    b = synth.band_1000_1500(2048)
+    x = noise(rng, 6000, 2048)
+    spec = make_spec(x, b, 100.0, 2.0, 64, method="subband", sbw=32, ndm0=16, tile_t=tile,
+                     use_zero_dm=True, zdm_c0=0, zdm_c1=2048)
+    got, ref, *_ = run_both(x, spec, want_masks=False)
+    assert np.array_equal(got, ref)
