@@ -199,6 +199,7 @@ enum LaunchKind : int {
   KB_DD_U8Q, KB_DD_U8, KB_DD_U8LDS, KB_DD_G_U8, KB_DD_G_U16, KB_DD_G_U32, KB_DD_G_F32F,
   KB_DD_G_F32D,
   KB_S2_V8, KB_S2_V16, KB_S2_C8, KB_S2_C16, KB_S2_U16, KB_S2_G_U16, KB_S2_G_U32, KB_S2_G_U64, KB_S2_G_F32,
+  KB_S2_F32C8, KB_S2_F32C16,
   KB_COPY,
   KB_COUNT
 };

@@ -917,6 +917,84 @@ __global__ __launch_bounds__(256) void k_stage2_generic(Stage2Args a) {
   stage2_generic_body<In, Acc, OutT>(a, blockIdx.x, gridDim.x);
 }
 
+// Stage 2 for float32 partial sums: a.sbc subbands at a time are staged in
+// LDS (tile + the group's halo, 16-byte loads), each wave keeps the FP64
+// sums of up to 4 fine DMs x P outputs per lane in registers across the
+// chunks, and stores float32.  One workgroup (512 threads) per (coarse DM,
+// tile of 64*P outputs); requires ndm0 <= 32.
+template <int P>
+__device__ __forceinline__ void stage2_f32c_body(const Stage2Args &a, uint32_t lid, uint32_t) {
+  extern __shared__ uint4 s2f[];
+  float *s2 = (float *)s2f;
+  constexpr int TILE = 64 * P;
+  constexpr int MD = 4;
+  const int ntile = (int)((a.nout + TILE - 1) / TILE);
+  const int ti = (int)(lid % (uint32_t)ntile);
+  const int j = (int)(lid / (uint32_t)ntile);
+  const size_t t0 = (size_t)ti * TILE;
+  const int ext4 = (TILE + a.maxres_j[j] + 3) / 4;     // 16-byte chunks per subband row
+  const int lds4 = a.lds_len / 4;
+  const float *__restrict__ src = (const float *)a.inter + (size_t)j * a.nsb * a.L1 + t0;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int nw = blockDim.x >> 6;
+  double acc[MD][P];
+#pragma unroll
+  for (int d = 0; d < MD; ++d)
+#pragma unroll
+    for (int i = 0; i < P; ++i) acc[d][i] = 0.0;
+  for (int sb0 = 0; sb0 < a.nsb; sb0 += a.sbc) {
+    const int nsc = min(a.sbc, a.nsb - sb0);
+    for (int idx = threadIdx.x; idx < nsc * ext4; idx += blockDim.x) {
+      const int sb = idx / ext4, k = idx % ext4;
+      s2f[sb * lds4 + k] = *(const uint4 *)(src + (size_t)(sb0 + sb) * a.L1 + 4 * k);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < MD; ++d) {
+      const int mi = wave + d * nw;
+      const int m = j * a.ndm0 + mi;
+      if (mi < a.ndm0 && m < a.ndm) {
+        const int32_t *__restrict__ rrow = a.res + (size_t)m * a.nsb + sb0;
+        for (int sb = 0; sb < nsc; ++sb) {
+          const float *p = s2 + sb * a.lds_len + lane * P + rrow[sb];
+#pragma unroll
+          for (int h = 0; h < P / 4; ++h) {
+            float4 v;
+            __builtin_memcpy(&v, p + 4 * h, 16);
+            acc[d][4 * h + 0] += (double)v.x;
+            acc[d][4 * h + 1] += (double)v.y;
+            acc[d][4 * h + 2] += (double)v.z;
+            acc[d][4 * h + 3] += (double)v.w;
+          }
+        }
+      }
+    }
+    __syncthreads();   // the next chunk overwrites LDS
+  }
+#pragma unroll
+  for (int d = 0; d < MD; ++d) {
+    const int mi = wave + d * nw;
+    const int m = j * a.ndm0 + mi;
+    if (mi >= a.ndm0 || m >= a.ndm) continue;
+    float *o = (float *)a.out + (size_t)m * a.ostride + t0 + (size_t)lane * P;
+    float r[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) r[i] = (float)acc[d][i];
+    if (t0 + (size_t)lane * P + P <= a.nout) {
+      __builtin_memcpy(o, r, 4 * P);
+    } else {
+      for (int i = 0; i < P; ++i)
+        if (t0 + (size_t)lane * P + i < a.nout) o[i] = r[i];
+    }
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(512) void k_stage2_f32c(Stage2Args a) {
+  stage2_f32c_body<P>(a, xcd_remap(blockIdx.x, gridDim.x), gridDim.x);
+}
+
 // ------------------------------------------------------ batched launches --
 // One launch runs the same stage of many plans: workgroup g (after the XCD
 // remap) belongs to argument block i with start[i] <= g < start[i+1] and runs
@@ -972,7 +1050,9 @@ AF_BATCHED(kb_s2_u16, Stage2Args, 256, (stage2_u16_body(a, lid, nb)))
 AF_BATCHED(kb_s2_g_u16, Stage2Args, 256, (stage2_generic_body<uint16_t, uint32_t, uint32_t>(a, lid, nb)))
 AF_BATCHED(kb_s2_g_u32, Stage2Args, 256, (stage2_generic_body<uint32_t, uint32_t, uint32_t>(a, lid, nb)))
 AF_BATCHED(kb_s2_g_u64, Stage2Args, 256, (stage2_generic_body<uint64_t, uint64_t, uint64_t>(a, lid, nb)))
-AF_BATCHED(kb_s2_g_f32, Stage2Args, 256, (stage2_generic_body<double, double, float>(a, lid, nb)))
+AF_BATCHED(kb_s2_g_f32, Stage2Args, 256, (stage2_generic_body<float, double, float>(a, lid, nb)))
+AF_BATCHED(kb_s2_f32c8, Stage2Args, 512, (stage2_f32c_body<8>(a, lid, nb)))
+AF_BATCHED(kb_s2_f32c16, Stage2Args, 512, (stage2_f32c_body<16>(a, lid, nb)))
 
 // ---------------------------------------------------------- launchers --
 
@@ -1072,7 +1152,9 @@ hipError_t launch_dedisperse(int dtype, const DdArgs &a, hipStream_t st) {
     return emit(KB_DD_G_U8, 7, nblk, 256, 0, a, st);
   case AF_U16: return emit(KB_DD_G_U16, 7, nblk, 256, 0, a, st);
   case AF_U32: return emit(KB_DD_G_U32, 7, nblk, 256, 0, a, st);
-  case AF_F32: return emit(a.ngrp == 1 ? KB_DD_G_F32F : KB_DD_G_F32D, 7, nblk, 256, 0, a, st);
+  // float32: FP64 accumulation, float32 stores (the output, or the subband
+  // partial sums)
+  case AF_F32: return emit(KB_DD_G_F32F, 7, nblk, 256, 0, a, st);
   default: return hipErrorInvalidValue;
   }
 }
@@ -1084,6 +1166,12 @@ hipError_t launch_stage2(int dtype, const Stage2Args &a, hipStream_t st) {
     const int ntile = (int)((a.nout + a.tile - 1) / a.tile);
     const size_t lds = (size_t)a.nsb * a.lds_len * sizeof(uint16_t);
     return emit(KB_S2_U16, 8, (size_t)ntile * a.ndm_nom, 256, lds, a, st);
+  }
+  if (dtype == AF_F32 && a.sbc > 0 && (a.tile == 512 || a.tile == 1024)) {
+    const int ntile = (int)((a.nout + a.tile - 1) / a.tile);
+    const size_t lds = (size_t)a.sbc * a.lds_len * sizeof(float);
+    return emit(a.tile == 1024 ? KB_S2_F32C16 : KB_S2_F32C8, 8, (size_t)ntile * a.ndm_nom, 512,
+                lds, a, st);
   }
   const size_t n = (size_t)a.ndm * a.nout;
   const size_t g = (n + 255) / 256;
@@ -1167,8 +1255,18 @@ hipError_t launch_desc(const LaunchDesc &d, hipStream_t st) {
                        argsof<Stage2Args>(d));
     break;
   case KB_S2_G_F32:
-    hipLaunchKernelGGL((k_stage2_generic<double, double, float>), g, b, d.lds, st,
+    hipLaunchKernelGGL((k_stage2_generic<float, double, float>), g, b, d.lds, st,
                        argsof<Stage2Args>(d));
+    break;
+  case KB_S2_F32C8:
+    (void)hipFuncSetAttribute((const void *)k_stage2_f32c<8>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)d.lds);
+    hipLaunchKernelGGL(k_stage2_f32c<8>, g, b, d.lds, st, argsof<Stage2Args>(d));
+    break;
+  case KB_S2_F32C16:
+    (void)hipFuncSetAttribute((const void *)k_stage2_f32c<16>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)d.lds);
+    hipLaunchKernelGGL(k_stage2_f32c<16>, g, b, d.lds, st, argsof<Stage2Args>(d));
     break;
   case KB_COPY: {
     const CopyArgs &c = argsof<CopyArgs>(d);
@@ -1223,6 +1321,16 @@ hipError_t launch_batched(int kind, const void *tab, const uint32_t *start, int 
     AF_BL(KB_S2_G_U32, kb_s2_g_u32, 256, Stage2Args)
     AF_BL(KB_S2_G_U64, kb_s2_g_u64, 256, Stage2Args)
     AF_BL(KB_S2_G_F32, kb_s2_g_f32, 256, Stage2Args)
+  case KB_S2_F32C8:
+    (void)hipFuncSetAttribute((const void *)kb_s2_f32c8, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    hipLaunchKernelGGL(kb_s2_f32c8, g, dim3(512), lds, st, (const Stage2Args *)tab, start, n);
+    break;
+  case KB_S2_F32C16:
+    (void)hipFuncSetAttribute((const void *)kb_s2_f32c16, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    hipLaunchKernelGGL(kb_s2_f32c16, g, dim3(512), lds, st, (const Stage2Args *)tab, start, n);
+    break;
 #undef AF_BL
   default:
     return launch_batched_u8(kind, tab, start, n, total, lds, st);

@@ -273,6 +273,20 @@ static af_status build(af_plan *p) {
           if (lds_of(t) <= (size_t)std::min(maxlds, 65536)) { tile = t; break; }
       }
     }
+    // float32 partial sums: LDS stage 2 with the subbands in chunks
+    if (d.dtype == AF_F32 && d.ndm0 <= 32) {
+      const int big[] = {512, 1024};
+      for (int t : big) {
+        if (d.tile_t != 0 && d.tile_t != t) continue;
+        const size_t row = (size_t)((t + p->maxres + 3) / 4 * 4) * 4;
+        const int sbc = (int)std::min<size_t>((size_t)p->nsb, 65536 / row);
+        if (sbc >= 2) {
+          tile = t;
+          p->s2_sbc = sbc;
+          break;
+        }
+      }
+    }
     // 0: stage 2 without LDS (u16 partial sums whose halo does not fit, or
     // wider partial sums)
     p->s2_tile = tile;
@@ -281,7 +295,7 @@ static af_status build(af_plan *p) {
     // Time super-tiles: the subband intermediate holds one super-tile of T2
     // outputs (plus the residual halo), at most kInterBudget bytes, as the
     // reference tiles time (src/core/dedispered.cu:37-39,1199-1235).
-    p->inter_elem = inter16 ? 2 : (d.dtype == AF_U32 || d.dtype == AF_F32 ? 8 : 4);
+    p->inter_elem = inter16 ? 2 : (d.dtype == AF_U32 ? 8 : 4);   // float32: float partials
     const size_t per_t = (size_t)p->ndm_nom * p->nsb * (size_t)p->inter_elem;
     const size_t budget = d.inter_budget ? d.inter_budget : kInterBudget;
     size_t tmax = budget / std::max<size_t>(per_t, 1);
