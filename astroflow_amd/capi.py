@@ -104,6 +104,7 @@ SIGNATURES = {
     "af_plan_stage_times": (_I, [_P, C.POINTER(StageTimes)]),
     "af_execute": (_I, [_P, _P, _P, _P, _P]),
     "af_execute_host": (_I, [_P, _P, _P, _P]),
+    "af_execute_spectrum": (_I, [_P, _P, _P, _S, _S, _P, _P]),
     "af_device_query": (_I, [_I, C.POINTER(DeviceProps)]),
     "af_device_count": (_I, [C.POINTER(_I)]),
     "af_set_device": (_I, [_I]),

@@ -327,6 +327,148 @@ py::object dedisperse_spec_raw(py::array_t<T, py::array::c_style> data, Header h
   return r.plane;
 }
 
+// ---------------------------------------------------- single-DM spectrum --
+// include/data.h:85-95
+struct SpectrumOut {
+  py::object data;              // [ntimes][nchans], the input's element type
+  size_t nchans = 0, ntimes = 0;
+  float tstart = 0, tend = 0, dm = 0;
+  int nbits = 0;
+  float freq_start = 0, freq_end = 0;
+};
+
+struct DevBuf {
+  void *p = nullptr;
+  explicit DevBuf(size_t n) { check(af_malloc(&p, n), "af_malloc"); }
+  ~DevBuf() { if (p) af_free(p); }
+};
+
+// cpucal::dedisperse_spec_with_dm<T> (include/cpucal.hpp:341-483): the time
+// window and channel selection as the reference derives them, the RFI
+// stages (IQRM over [chan_start, chan_end_excl), window masks, static mask;
+// no zero-DM) on the local slice the reference cleans, then every channel
+// read at its delay for this DM.  The input array is not modified (the
+// reference cleans it in place).
+template <typename T>
+SpectrumOut spectrum_at_dm(py::array_t<T, py::array::c_style> data, Header h, float tstart,
+                           float tend, float dm, float freq_start, float freq_end,
+                           std::string maskfile, RFIConfig rfi) {
+  const py::buffer_info bi = data.request();
+  if (tend <= tstart) throw py::value_error("tend must be > tstart");
+  if (h.nchans <= 0 || h.ndata <= 0) throw py::value_error("empty header");
+  if ((size_t)bi.size < (size_t)h.ndata * (size_t)h.nchans)
+    throw py::value_error("data holds fewer than header.ndata * header.nchans samples");
+  const size_t ndata = (size_t)h.ndata, nchans = (size_t)h.nchans;
+  const size_t t_start = (size_t)(tstart / h.tsamp), t_end = (size_t)(tend / h.tsamp);
+  const size_t t_len_req = t_end > t_start ? t_end - t_start : 0;
+  if (t_len_req == 0 || t_start >= ndata) throw py::value_error("Invalid time window for this file.");
+  if (freq_start >= freq_end) throw py::value_error("freq_end must be > freq_start");
+  if (dm < 0.0f) throw py::value_error("dm must be >= 0");
+  if (h.foff < 0)
+    throw py::value_error("frequency channels are in descending order, which is not supported yet.");
+  std::vector<double> freq(nchans);
+  std::vector<float> ff(nchans);
+  for (size_t i = 0; i < nchans; ++i) {
+    ff[i] = h.fch1 + (float)i * h.foff;
+    freq[i] = ff[i];
+  }
+  const float f0 = ff.front(), fN = ff.back();
+  const float fmin = std::min(f0, fN), fmax = std::max(f0, fN);
+  if (freq_start < fmin || freq_end > fmax) {
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), "Frequency range [%.3f, %.3f] MHz out of file range [%.3f, %.3f] MHz",
+                  freq_start, freq_end, fmin, fmax);
+    throw py::value_error(buf);
+  }
+  auto to_index = [&](float f) -> size_t {
+    const double pos = (double)(f - f0) / (double)(fN - f0);
+    double raw = pos * (double)(nchans - 1);
+    raw = std::min(std::max(raw, 0.0), (double)(nchans - 1));
+    return (size_t)std::floor(raw + 1e-9);
+  };
+  size_t cs = to_index(freq_start), ce = to_index(freq_end);
+  if (ce < cs) std::swap(cs, ce);
+  const size_t cx = std::min(ce + 1, nchans);
+  const size_t sel = cx > cs ? cx - cs : 0;
+  if (sel == 0) throw py::value_error("Empty channel selection.");
+  const double f_high = std::max(freq[cs], freq[cx - 1]);
+  int ndev = 0;
+  if (af_device_count(&ndev) != AF_OK || ndev == 0) throw std::runtime_error("No HIP devices found");
+  // delays first (a one-sample probe plan gives max_delay), then the slice
+  af_plan_desc d{};
+  d.dtype = dtype_code<T>();
+  d.nchan = (int)nchans;
+  d.ds = 1;
+  d.tsamp = h.tsamp;
+  d.freq_mhz = freq.data();
+  d.f_ref_mhz = f_high;
+  d.dd_c0 = (int)cs;
+  d.dd_c1 = (int)cx;
+  d.dm_low = dm;
+  d.dm_step = 1.0;
+  d.ndm = 1;
+  d.method = AF_METHOD_DIRECT;
+  d.nsamp = 1;   // the delays do not depend on the length
+  af_plan *probe = nullptr;
+  check(af_plan_create(&d, &probe), "af_plan_create");
+  af_plan_info pi;
+  af_plan_query(probe, &pi);
+  af_plan_destroy(probe);
+  const size_t dmax = (size_t)pi.max_delay;
+  const size_t cap = ndata - t_start;
+  const size_t t_len = cap > dmax ? std::min(t_len_req, cap - dmax) : 0;
+  if (t_len == 0) throw py::value_error("Time window too short for this DM and band.");
+  const size_t r0 = t_start > dmax ? t_start - dmax : 0;
+  const size_t off = t_start - r0;
+  const size_t slen = std::min(t_len + off + 2 * dmax, ndata - r0);
+  std::vector<int> bad;
+  if (rfi.use_mask) bad = read_mask_file(maskfile);
+  d.nsamp = slen;
+  d.use_iqrm = rfi.use_iqrm;
+  d.iqrm_c0 = (int)cs;   // this path passes the exclusive end (include/cpucal.hpp:425-430)
+  d.iqrm_c1 = (int)cx;
+  d.iqrm_mode = rfi.iqrm_cfg.mode;
+  d.iqrm_radius_frac = rfi.iqrm_cfg.radius_frac;
+  d.iqrm_nsigma = rfi.iqrm_cfg.nsigma;
+  d.iqrm_geofactor = rfi.iqrm_cfg.geofactor;
+  const double tb = (double)h.tsamp;
+  const double win = std::max(0.0, rfi.iqrm_cfg.win_sec), hop = std::max(0.0, rfi.iqrm_cfg.hop_sec);
+  const unsigned W = win > 0.0 ? (unsigned)std::floor(win / tb) : 0u;
+  const unsigned H = hop > 0.0 ? (unsigned)std::floor(hop / tb) : W;
+  d.iqrm_win = (W >= slen) ? 0u : W;
+  d.iqrm_hop = H;
+  d.iqrm_include_tail = rfi.iqrm_cfg.include_tail;
+  d.use_zero_dm = 0;
+  d.bad_chans = bad.empty() ? nullptr : bad.data();
+  d.n_bad = (int)bad.size();
+  af_plan *plan = nullptr;
+  check(af_plan_create(&d, &plan), "af_plan_create");
+  std::unique_ptr<af_plan, decltype(&af_plan_destroy)> guard(plan, af_plan_destroy);
+  py::array_t<T> out({(py::ssize_t)t_len, (py::ssize_t)sel});
+  const T *src = static_cast<const T *>(bi.ptr) + r0 * nchans;
+  T *dst = out.mutable_data();
+  af_status st;
+  {
+    py::gil_scoped_release nogil;
+    DevBuf din(slen * nchans * sizeof(T)), dout(t_len * sel * sizeof(T));
+    st = af_memcpy_h2d(din.p, src, slen * nchans * sizeof(T), nullptr);
+    if (st == AF_OK) st = af_execute_spectrum(plan, din.p, dout.p, off, t_len, nullptr, nullptr);
+    if (st == AF_OK) st = af_memcpy_d2h(dst, dout.p, t_len * sel * sizeof(T), nullptr);
+  }
+  check(st, "af_execute_spectrum");
+  SpectrumOut r;
+  r.data = out;
+  r.ntimes = t_len;
+  r.nchans = sel;
+  r.tstart = tstart;
+  r.tend = tstart + (float)t_len * h.tsamp;
+  r.dm = dm;
+  r.nbits = h.nbits;
+  r.freq_start = std::min(ff[cs], ff[cx - 1]);
+  r.freq_end = std::max(ff[cs], ff[cx - 1]);
+  return r;
+}
+
 // ---------------------------------------------------- filterbank ingest --
 // SIGPROC filterbank: a keyword header between HEADER_START and HEADER_END
 // (each keyword an int32 length + text, followed by its value), then
@@ -455,12 +597,25 @@ DedispersedDataUint8 dedispered_fil(std::string filename, float dm_low, float dm
 }
 
 template <typename T>
+SpectrumOut fil_spectrum_at_dm(const Filterbank &fil, float tstart, float tend, float dm,
+                               float freq_start, float freq_end, std::string maskfile,
+                               RFIConfig rfi) {
+  if (fil.nbits != (int)sizeof(T) * 8) throw py::value_error("filterbank nbits does not match");
+  py::array a = fil.data();
+  return spectrum_at_dm<T>(a.cast<py::array_t<T, py::array::c_style>>(), fil.header(), tstart,
+                           tend, dm, freq_start, freq_end, maskfile, rfi);
+}
+
+template <typename T>
 void def_spec(py::module_ &m) {
   m.def("_dedisperse_spec", &dedisperse_spec<T>, py::arg("data"), py::arg("header"),
         py::arg("dm_low"), py::arg("dm_high"), py::arg("freq_start"), py::arg("freq_end"),
         py::arg("dm_step") = 1.0f, py::arg("time_downsample") = 2, py::arg("t_sample") = 0.5f,
         py::arg("target_id") = 0, py::arg("mask_file") = "mask.txt",
         py::arg("rficfg") = RFIConfig());
+  m.def("_dedisperse_spec_with_dm", &spectrum_at_dm<T>, py::arg("data"), py::arg("header"),
+        py::arg("tstart"), py::arg("tend"), py::arg("dm"), py::arg("freq_start"),
+        py::arg("freq_end"), py::arg("maskfile") = "mask.txt", py::arg("rficfg") = RFIConfig());
   m.def("_dedisperse_spec_raw", &dedisperse_spec_raw<T>, py::arg("data"), py::arg("header"),
         py::arg("dm_low"), py::arg("dm_high"), py::arg("freq_start"), py::arg("freq_end"),
         py::arg("dm_step") = 1.0f, py::arg("time_downsample") = 1, py::arg("target_id") = 0,
@@ -539,6 +694,32 @@ PYBIND11_MODULE(_astroflow_core, m) {
       .def_readonly("ndata", &Filterbank::ndata)
       .def_property_readonly("data", &Filterbank::data)
       .def_property_readonly("header", &Filterbank::header);
+
+  py::class_<SpectrumOut>(m, "Spectrum")
+      .def(py::init<>())
+      .def_readonly("data", &SpectrumOut::data)
+      .def_readonly("ntimes", &SpectrumOut::ntimes)
+      .def_readonly("nchans", &SpectrumOut::nchans)
+      .def_readonly("tstart", &SpectrumOut::tstart)
+      .def_readonly("tend", &SpectrumOut::tend)
+      .def_readonly("dm", &SpectrumOut::dm)
+      .def_readonly("nbits", &SpectrumOut::nbits)
+      .def_readonly("freq_start", &SpectrumOut::freq_start)
+      .def_readonly("freq_end", &SpectrumOut::freq_end);
+  // the reference's per-type names (src/bindpy/bindings.cpp:22-24) share one class
+  m.attr("Spectrum8") = m.attr("Spectrum");
+  m.attr("Spectrum16") = m.attr("Spectrum");
+  m.attr("Spectrum32") = m.attr("Spectrum");
+
+  m.def("_dedispered_fil_with_dm_uint8", &fil_spectrum_at_dm<uint8_t>, py::arg("fil"),
+        py::arg("tstart"), py::arg("tend"), py::arg("dm"), py::arg("freq_start"),
+        py::arg("freq_end"), py::arg("maskfile"), py::arg("rficfg"));
+  m.def("_dedispered_fil_with_dm_uint16", &fil_spectrum_at_dm<uint16_t>, py::arg("fil"),
+        py::arg("tstart"), py::arg("tend"), py::arg("dm"), py::arg("freq_start"),
+        py::arg("freq_end"), py::arg("maskfile"), py::arg("rficfg"));
+  m.def("_dedispered_fil_with_dm_uint32", &fil_spectrum_at_dm<uint32_t>, py::arg("fil"),
+        py::arg("tstart"), py::arg("tend"), py::arg("dm"), py::arg("freq_start"),
+        py::arg("freq_end"), py::arg("maskfile"), py::arg("rficfg"));
 
   // float32 first: pybind11 tries exact dtype matches before converting, and
   // a float array must not be silently converted to an integer overload

@@ -138,6 +138,9 @@ struct Stage2Args {
   int sbc;                  // u16 LDS kernel: subbands staged at a time (0 = all)
 };
 hipError_t launch_stage2(int dtype, const Stage2Args &a, hipStream_t st);
+// Single-DM spectrum: out[t][c] = xT[c][t0 + t + delay[c]], [tlen][nch].
+hipError_t launch_gather(int dtype, const void *xT, size_t Lp, const int32_t *delay, int nch,
+                         size_t t0, size_t tlen, void *out, hipStream_t st);
 // Device-to-device copy on `st` (recorded like the kernels, see below).
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st);
 

@@ -995,6 +995,32 @@ __global__ __launch_bounds__(512) void k_stage2_f32c(Stage2Args a) {
   stage2_f32c_body<P>(a, xcd_remap(blockIdx.x, gridDim.x), gridDim.x);
 }
 
+// --------------------------------------------------- single-DM spectrum --
+// include/cpucal.hpp:467-481: every channel of the cleaned data read at its
+// delay.  A workgroup moves a 64-channel x 64-sample block through LDS:
+// channel rows are read contiguously, time rows written contiguously.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather(const T *__restrict__ xT, size_t Lp,
+                                                const int32_t *__restrict__ delay, int nch,
+                                                size_t t0, size_t tlen, T *__restrict__ out) {
+  __shared__ T tile[64][65];
+  const int cb = blockIdx.y * 64;
+  const size_t tb = (size_t)blockIdx.x * 64;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int k = w; k < 64; k += 4) {
+    const int c = cb + k;
+    T v = 0;
+    if (c < nch && tb + lane < tlen) v = xT[(size_t)c * Lp + t0 + tb + lane + (size_t)delay[c]];
+    tile[k][lane] = v;
+  }
+  __syncthreads();
+  for (int k = w; k < 64; k += 4) {
+    const size_t t = tb + k;
+    const int c = cb + lane;
+    if (t < tlen && c < nch) out[t * (size_t)nch + c] = tile[lane][k];
+  }
+}
+
 // ------------------------------------------------------ batched launches --
 // One launch runs the same stage of many plans: workgroup g (after the XCD
 // remap) belongs to argument block i with start[i] <= g < start[i+1] and runs
@@ -1077,6 +1103,15 @@ static hipError_t emit(int kind, int stage, size_t nblk, unsigned threads, size_
                        hipStream_t st) {
   return emit_args(kind, stage, nblk, threads, lds, args, st);
 }
+
+#define AF_DISPATCH_T(dtype, T, ...)                                          \
+  switch (dtype) {                                                            \
+  case AF_U8: { using T = uint8_t; __VA_ARGS__; break; }                      \
+  case AF_U16: { using T = uint16_t; __VA_ARGS__; break; }                    \
+  case AF_U32: { using T = uint32_t; __VA_ARGS__; break; }                    \
+  case AF_F32: { using T = float; __VA_ARGS__; break; }                       \
+  default: return hipErrorInvalidValue;                                       \
+  }
 
 #define AF_KIND4(base, dtype)                                                    \
   ((dtype) == AF_U8 ? base##U8 : (dtype) == AF_U16 ? base##U16 : (dtype) == AF_U32 ? base##U32 \
@@ -1183,6 +1218,18 @@ hipError_t launch_stage2(int dtype, const Stage2Args &a, hipStream_t st) {
   case AF_F32: return emit(KB_S2_G_F32, 8, g, 256, 0, a, st);
   default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_gather(int dtype, const void *xT, size_t Lp, const int32_t *delay, int nch,
+                         size_t t0, size_t tlen, void *out, hipStream_t st) {
+  const size_t gx = (tlen + 63) / 64, gy = ((size_t)nch + 63) / 64;
+  if (gx == 0 || gy == 0) return hipSuccess;
+  if (gx > 0x7FFFFFFFull || gy > 65535) return hipErrorInvalidValue;
+  const dim3 g((unsigned)gx, (unsigned)gy), b(256);
+  AF_DISPATCH_T(dtype, T,
+                hipLaunchKernelGGL(k_gather<T>, g, b, 0, st, (const T *)xT, Lp, delay, nch, t0,
+                                   tlen, (T *)out));
+  return hipGetLastError();
 }
 
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st) {

@@ -20,6 +20,16 @@ __device__ __forceinline__ uint4 ld16(const uint8_t *p) {
   return v;
 }
 
+// acc + low / high 16-bit half of v in one v_dot2_u32_u16 (a dot product
+// with (1,0) or (0,1)): unpack and add in a single instruction.
+typedef unsigned short u16x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t add_lo16(uint32_t acc, uint32_t v) {
+  return __builtin_amdgcn_udot2(__builtin_bit_cast(u16x2_t, v), (u16x2_t){1, 0}, acc, false);
+}
+__device__ __forceinline__ uint32_t add_hi16(uint32_t acc, uint32_t v) {
+  return __builtin_amdgcn_udot2(__builtin_bit_cast(u16x2_t, v), (u16x2_t){0, 1}, acc, false);
+}
+
 __device__ __forceinline__ uint32_t u4get(const uint4 &v, int k) {
   return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
 }
@@ -369,10 +379,10 @@ __device__ __forceinline__ void stage2_u16v_body(const Stage2Args &a, uint32_t l
       for (int h = 0; h < P / 8; ++h) {
         uint4 v;
         __builtin_memcpy(&v, p + 8 * h, 16);
-        acc[8 * h + 0] += v.x & 0xFFFFu; acc[8 * h + 1] += v.x >> 16;
-        acc[8 * h + 2] += v.y & 0xFFFFu; acc[8 * h + 3] += v.y >> 16;
-        acc[8 * h + 4] += v.z & 0xFFFFu; acc[8 * h + 5] += v.z >> 16;
-        acc[8 * h + 6] += v.w & 0xFFFFu; acc[8 * h + 7] += v.w >> 16;
+        acc[8 * h + 0] = add_lo16(acc[8 * h + 0], v.x); acc[8 * h + 1] = add_hi16(acc[8 * h + 1], v.x);
+        acc[8 * h + 2] = add_lo16(acc[8 * h + 2], v.y); acc[8 * h + 3] = add_hi16(acc[8 * h + 3], v.y);
+        acc[8 * h + 4] = add_lo16(acc[8 * h + 4], v.z); acc[8 * h + 5] = add_hi16(acc[8 * h + 5], v.z);
+        acc[8 * h + 6] = add_lo16(acc[8 * h + 6], v.w); acc[8 * h + 7] = add_hi16(acc[8 * h + 7], v.w);
       }
     }
     uint32_t *o = (uint32_t *)a.out + (size_t)m * a.ostride + t0 + (size_t)lane * P;
@@ -434,10 +444,14 @@ __device__ __forceinline__ void stage2_u16c_body(const Stage2Args &a, uint32_t l
           for (int h = 0; h < P / 8; ++h) {
             uint4 v;
             __builtin_memcpy(&v, p + 8 * h, 16);
-            acc[d][8 * h + 0] += v.x & 0xFFFFu; acc[d][8 * h + 1] += v.x >> 16;
-            acc[d][8 * h + 2] += v.y & 0xFFFFu; acc[d][8 * h + 3] += v.y >> 16;
-            acc[d][8 * h + 4] += v.z & 0xFFFFu; acc[d][8 * h + 5] += v.z >> 16;
-            acc[d][8 * h + 6] += v.w & 0xFFFFu; acc[d][8 * h + 7] += v.w >> 16;
+            acc[d][8 * h + 0] = add_lo16(acc[d][8 * h + 0], v.x);
+            acc[d][8 * h + 1] = add_hi16(acc[d][8 * h + 1], v.x);
+            acc[d][8 * h + 2] = add_lo16(acc[d][8 * h + 2], v.y);
+            acc[d][8 * h + 3] = add_hi16(acc[d][8 * h + 3], v.y);
+            acc[d][8 * h + 4] = add_lo16(acc[d][8 * h + 4], v.z);
+            acc[d][8 * h + 5] = add_hi16(acc[d][8 * h + 5], v.z);
+            acc[d][8 * h + 6] = add_lo16(acc[d][8 * h + 6], v.w);
+            acc[d][8 * h + 7] = add_hi16(acc[d][8 * h + 7], v.w);
           }
         }
       }

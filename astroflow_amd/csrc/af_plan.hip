@@ -525,13 +525,10 @@ static af_status collect(af_plan *p) {
   return AF_OK;
 }
 
-af_status execute(af_plan *p, const void *d_in, void *d_out, uint8_t *d_masks, hipStream_t st) {
+// Binning, IQRM, window masks, zero-DM and static mask into the channel-
+// major buffer p->d_xT (src/core/dedispered.cu:1050-1118).
+static af_status run_clean(af_plan *p, const void *d_in, hipStream_t st, int set) {
   const af_plan_desc &d = p->d;
-  int set = -1;
-  if (p->profiling) {
-    if ((size_t)(p->ev_used + 1) * af_plan::kNumEv > p->ev.size()) AF_TRY(collect(p));
-    set = p->ev_used++;
-  }
   AF_TRY(record(p, set, 0, st));
   const void *src = d_in;
   if (d.ds > 1) {
@@ -564,8 +561,20 @@ af_status execute(af_plan *p, const void *d_in, void *d_out, uint8_t *d_masks, h
   ca.dd_c1 = d.dd_c1;
   AF_HIP(launch_clean(d.dtype, ca, st));
   AF_TRY(record(p, set, 4, st));
+  return AF_OK;
+}
+
+af_status execute(af_plan *p, const void *d_in, void *d_out, uint8_t *d_masks, hipStream_t st) {
+  const af_plan_desc &d = p->d;
+  int set = -1;
+  if (p->profiling) {
+    if ((size_t)(p->ev_used + 1) * af_plan::kNumEv > p->ev.size()) AF_TRY(collect(p));
+    set = p->ev_used++;
+  }
+  AF_TRY(run_clean(p, d_in, st, set));
   AF_TRY(run_dedisp(p, p->d_xT, p->Lp, d_out, st, set));
-  if (d_masks && iq) AF_HIP(launch_copy(d_masks, p->d_masks, (size_t)p->nwin * d.nchan, st));
+  if (d_masks && d.use_iqrm && p->nwin > 0)
+    AF_HIP(launch_copy(d_masks, p->d_masks, (size_t)p->nwin * d.nchan, st));
   return AF_OK;
 }
 
@@ -755,6 +764,23 @@ af_status af_execute(af_plan *p, const void *d_in, void *d_out, uint8_t *d_masks
   if (!p || !d_in || !d_out) return fail(AF_ERR_INVALID, "null argument");
   AF_HIP(hipSetDevice(p->device));
   return execute(p, d_in, d_out, d_masks, (hipStream_t)stream);
+}
+
+af_status af_execute_spectrum(af_plan *p, const void *d_in, void *d_out, size_t t0, size_t t_len,
+                              uint8_t *d_masks, void *stream) {
+  if (!p || !d_in || !d_out) return fail(AF_ERR_INVALID, "null argument");
+  const af_plan_desc &d = p->d;
+  if (d.method != AF_METHOD_DIRECT || p->ndm != 1 || d.ds != 1)
+    return fail(AF_ERR_INVALID, "a spectrum plan is direct, with one DM and ds == 1");
+  if (t_len == 0 || t0 + t_len + (size_t)p->maxdelay > p->nbin)
+    return fail(AF_ERR_INVALID, "spectrum window reaches beyond the input");
+  AF_HIP(hipSetDevice(p->device));
+  hipStream_t st = (hipStream_t)stream;
+  AF_TRY(run_clean(p, d_in, st, -1));
+  AF_HIP(launch_gather(d.dtype, p->d_xT, p->Lp, p->d_delays, p->nch, t0, t_len, d_out, st));
+  if (d_masks && d.use_iqrm && p->nwin > 0)
+    AF_HIP(launch_copy(d_masks, p->d_masks, (size_t)p->nwin * d.nchan, st));
+  return AF_OK;
 }
 
 af_status af_execute_host(af_plan *p, const void *h_in, void *h_out, uint8_t *h_masks) {

@@ -132,6 +132,16 @@ af_status af_plan_stage_times(af_plan *plan, af_stage_times *t);
 af_status af_execute(af_plan *plan, const void *d_in, void *d_out,
                      uint8_t *d_masks, void *stream);
 
+/* Single-DM spectrum (replaces cpucal::dedisperse_spec_with_dm<T>,
+ * include/cpucal.hpp:341-483): the plan's RFI stages on d_in, then
+ *   d_out[t][c - dd_c0] = cleaned[t0 + t + delay(c)][c],  t < t_len,
+ * channels [dd_c0, dd_c1), delay(c) the plan's delay of its one DM
+ * (a direct plan with ndm == 1 and ds == 1; f_ref the highest selected
+ * channel).  d_out has the input's element type, [t_len][dd_c1-dd_c0].
+ * t0 + t_len + max_delay must not exceed nsamp. */
+af_status af_execute_spectrum(af_plan *plan, const void *d_in, void *d_out, size_t t0,
+                              size_t t_len, uint8_t *d_masks, void *stream);
+
 /* Same on host buffers: upload, execute, download, synchronise. */
 af_status af_execute_host(af_plan *plan, const void *h_in, void *h_out,
                           uint8_t *h_masks);

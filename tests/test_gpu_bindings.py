@@ -122,3 +122,69 @@ def test_filterbank_call_equals_array_call(gpu_lib, tmp_path):
 def test_out_of_band_range_is_value_error(gpu_lib):
     with pytest.raises(ValueError, match="out of spectrum range"):
         core._dedisperse_spec_raw(data_u8(), header(), 0.0, 10.0, 1100.0, 1400.0)
+
+
+def spectrum_oracle(x, tstart, tend, dm, fs, fe, rfi, bad):
+    """cpucal::dedisperse_spec_with_dm with the oracle's pieces (FP64 delays)."""
+    freq = O.freq_table_f32(float(FCH1), float(FOFF), NCH)
+    ff = freq.astype(np.float32)
+    t0 = int(np.float32(tstart) / TSAMP)
+    t1 = int(np.float32(tend) / TSAMP)
+    f0, fN = ff[0], ff[-1]
+
+    def idx(f):
+        raw = float(np.float32(f) - f0) / float(fN - f0) * (NCH - 1)
+        raw = min(max(raw, 0.0), NCH - 1)
+        return int(np.floor(raw + 1e-9))
+    cs, ce = sorted((idx(fs), idx(fe)))
+    cx = min(ce + 1, NCH)
+    d = O.delay_table(freq, cs, cx, max(freq[cs], freq[cx - 1]), float(TSAMP), 1,
+                      np.array([float(np.float32(dm))]))[0]
+    dmax = int(d.max())
+    tl = min(t1 - t0, NSAMP - t0 - dmax)
+    r0 = max(t0 - dmax, 0)
+    off = t0 - r0
+    slen = min(tl + off + 2 * dmax, NSAMP - r0)
+    sl = x[r0:r0 + slen].copy()
+    if rfi.use_iqrm:
+        W = int(np.floor(rfi.iqrm_cfg.win_sec / float(TSAMP))) if rfi.iqrm_cfg.win_sec > 0 else 0
+        H = int(np.floor(rfi.iqrm_cfg.hop_sec / float(TSAMP))) if rfi.iqrm_cfg.hop_sec > 0 else W
+        m, a0, a1 = O.iqrm(sl, cs, cx, rfi.iqrm_cfg.mode, rfi.iqrm_cfg.radius_frac,
+                           rfi.iqrm_cfg.nsigma, rfi.iqrm_cfg.geofactor, 0 if W >= slen else W, H,
+                           rfi.iqrm_cfg.include_tail, O.IQRM_GPU)
+        O.apply_window_masks(sl, m, a0, a1)
+    if rfi.use_mask and bad:
+        O.static_mask(sl, bad)
+    out = np.empty((tl, cx - cs), x.dtype)
+    for c in range(cs, cx):
+        out[:, c - cs] = sl[off + d[c - cs]: off + d[c - cs] + tl, c]
+    return out, cs, cx
+
+
+def test_spectrum_at_dm_matches_oracle(gpu_lib, tmp_path):
+    say("case: _dedisperse_spec_with_dm u8, IQRM + mask file, DM 80")
+    x = data_u8(8)
+    mask = tmp_path / "mask.txt"
+    mask.write_text("20 21 22")
+    rfi = core.RFIconfig(True, False, True, core.IQRMconfig(1, 0.03, 3.0, 1.5, 0.0, 0.0, True))
+    args = (0.05, 0.45, 80.0, 1230.0, 1480.0)
+    sp = core._dedisperse_spec_with_dm(x, header(), *args, str(mask), rfi)
+    ref, cs, cx = spectrum_oracle(x, *args, rfi, [20, 21, 22])
+    assert sp.data.shape == ref.shape and sp.data.dtype == np.uint8
+    assert (sp.ntimes, sp.nchans) == ref.shape
+    assert np.array_equal(sp.data, ref)
+    # the input array is left as it was
+    assert np.array_equal(x, data_u8(8))
+
+
+def test_spectrum_at_dm_zero_is_the_input(gpu_lib):
+    # python/tests/test_dedispered.py:94-101: at DM 0 without RFI the
+    # spectrum is the input's selected channels
+    say("case: _dedisperse_spec_with_dm at DM 0")
+    x = data_u8(9)
+    sp = core._dedisperse_spec_with_dm(x, header(), 0.0, float(NSAMP * TSAMP), 0.0, 1210.0,
+                                       1490.0, "none", core.RFIconfig())
+    _, cs, cx = spectrum_oracle(x, 0.0, float(NSAMP * TSAMP), 0.0, 1210.0, 1490.0,
+                                core.RFIconfig(), [])
+    assert sp.data.shape == (NSAMP, cx - cs)
+    assert np.array_equal(sp.data, x[:, cs:cx])
