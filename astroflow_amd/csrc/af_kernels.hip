@@ -65,6 +65,33 @@ __host__ __device__ inline double u128_to_double(uint64_t hi, uint64_t lo) {
   return ldexp((double)keep, 11 + sh);
 }
 
+// Copies nrows rows of `ext` 16-byte chunks (row r starts at src + r*pitch
+// elements of E per chunk) into LDS rows of `ldsw` chunks.  Each thread keeps
+// four loads in flight before its LDS stores.
+template <typename E>
+__device__ __forceinline__ void fill_rows16(uint4 *__restrict__ lds, int ldsw,
+                                            const E *__restrict__ src, size_t pitch, int epc,
+                                            int nrows, int ext) {
+  const int total = nrows * ext;
+  const int step = (int)blockDim.x;
+  for (int base = threadIdx.x; base < total; base += 4 * step) {
+    uint4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int idx = base + u * step;
+      if (idx < total) {
+        const int r = idx / ext, k = idx % ext;
+        v[u] = *(const uint4 *)(src + (size_t)r * pitch + (size_t)epc * k);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int idx = base + u * step;
+      if (idx < total) lds[(idx / ext) * ldsw + idx % ext] = v[u];
+    }
+  }
+}
+
 // ------------------------------------------------------------------ binning --
 // src/core/dedispered.cu:337-376 (sqrt compression of the ds-sample sum).
 template <typename T>
@@ -945,10 +972,7 @@ __device__ __forceinline__ void stage2_f32c_body(const Stage2Args &a, uint32_t l
     for (int i = 0; i < P; ++i) acc[d][i] = 0.0;
   for (int sb0 = 0; sb0 < a.nsb; sb0 += a.sbc) {
     const int nsc = min(a.sbc, a.nsb - sb0);
-    for (int idx = threadIdx.x; idx < nsc * ext4; idx += blockDim.x) {
-      const int sb = idx / ext4, k = idx % ext4;
-      s2f[sb * lds4 + k] = *(const uint4 *)(src + (size_t)(sb0 + sb) * a.L1 + 4 * k);
-    }
+    fill_rows16(s2f, lds4, src + (size_t)sb0 * a.L1, a.L1, 4, nsc, ext4);
     __syncthreads();
 #pragma unroll
     for (int d = 0; d < MD; ++d) {
@@ -957,16 +981,9 @@ __device__ __forceinline__ void stage2_f32c_body(const Stage2Args &a, uint32_t l
       if (mi < a.ndm0 && m < a.ndm) {
         const int32_t *__restrict__ rrow = a.res + (size_t)m * a.nsb + sb0;
         for (int sb = 0; sb < nsc; ++sb) {
-          const float *p = s2 + sb * a.lds_len + lane * P + rrow[sb];
+          const float *p = s2 + sb * a.lds_len + lane + rrow[sb];   // aligned 4-byte reads
 #pragma unroll
-          for (int h = 0; h < P / 4; ++h) {
-            float4 v;
-            __builtin_memcpy(&v, p + 4 * h, 16);
-            acc[d][4 * h + 0] += (double)v.x;
-            acc[d][4 * h + 1] += (double)v.y;
-            acc[d][4 * h + 2] += (double)v.z;
-            acc[d][4 * h + 3] += (double)v.w;
-          }
+          for (int i = 0; i < P; ++i) acc[d][i] += (double)p[64 * i];
         }
       }
     }
@@ -977,16 +994,10 @@ __device__ __forceinline__ void stage2_f32c_body(const Stage2Args &a, uint32_t l
     const int mi = wave + d * nw;
     const int m = j * a.ndm0 + mi;
     if (mi >= a.ndm0 || m >= a.ndm) continue;
-    float *o = (float *)a.out + (size_t)m * a.ostride + t0 + (size_t)lane * P;
-    float r[P];
+    float *o = (float *)a.out + (size_t)m * a.ostride + t0 + (size_t)lane;
 #pragma unroll
-    for (int i = 0; i < P; ++i) r[i] = (float)acc[d][i];
-    if (t0 + (size_t)lane * P + P <= a.nout) {
-      __builtin_memcpy(o, r, 4 * P);
-    } else {
-      for (int i = 0; i < P; ++i)
-        if (t0 + (size_t)lane * P + i < a.nout) o[i] = r[i];
-    }
+    for (int i = 0; i < P; ++i)
+      if (t0 + (size_t)lane + 64 * i < a.nout) o[64 * i] = (float)acc[d][i];
   }
 }
 

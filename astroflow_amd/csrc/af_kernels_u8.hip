@@ -44,6 +44,33 @@ __device__ __forceinline__ uint4 load_row16(const uint8_t *row, int cb, int lim)
   return make_uint4(w[0], w[1], w[2], w[3]);
 }
 
+// Copies nrows rows of `ext` 16-byte chunks (row r starts at src + r*pitch
+// elements of E per chunk) into LDS rows of `ldsw` chunks.  Each thread keeps
+// four loads in flight before its LDS stores.
+template <typename E>
+__device__ __forceinline__ void fill_rows16(uint4 *__restrict__ lds, int ldsw,
+                                            const E *__restrict__ src, size_t pitch, int epc,
+                                            int nrows, int ext) {
+  const int total = nrows * ext;
+  const int step = (int)blockDim.x;
+  for (int base = threadIdx.x; base < total; base += 4 * step) {
+    uint4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int idx = base + u * step;
+      if (idx < total) {
+        const int r = idx / ext, k = idx % ext;
+        v[u] = *(const uint4 *)(src + (size_t)r * pitch + (size_t)epc * k);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int idx = base + u * step;
+      if (idx < total) lds[(idx / ext) * ldsw + idx % ext] = v[u];
+    }
+  }
+}
+
 // ------------------------------------------------------ IQRM statistics --
 // Per (row block <= 1024 rows, channel): sum and sum of squares.  A 1024-
 // thread block covers 4096 channels (16 per thread) of every fourth row per
@@ -345,7 +372,7 @@ __global__ __launch_bounds__(1024) void k_clean_u8(CleanArgs a) {
 // One workgroup (512 threads) per (coarse DM group, tile of 64*P outputs):
 // the group's partial sums for the tile plus its largest residual are staged
 // in LDS with 16-byte loads; each wave then sums the subbands of one fine DM
-// at a time, each lane P consecutive outputs read as P/8 16-byte LDS loads.
+// at a time, lane l of a wave summing outputs l, l+64, ..., l+64(P-1).
 template <int P>
 __device__ __forceinline__ void stage2_u16v_body(const Stage2Args &a, uint32_t lid, uint32_t) {
   extern __shared__ uint4 s2v[];
@@ -358,10 +385,7 @@ __device__ __forceinline__ void stage2_u16v_body(const Stage2Args &a, uint32_t l
   const int ext8 = (TILE + a.maxres_j[j] + 7) / 8;   // 16-byte chunks per subband row
   const int lds8 = a.lds_len / 8;
   const uint16_t *__restrict__ src = (const uint16_t *)a.inter + (size_t)j * a.nsb * a.L1 + t0;
-  for (int idx = threadIdx.x; idx < a.nsb * ext8; idx += blockDim.x) {
-    const int sb = idx / ext8, k = idx % ext8;
-    s2v[sb * lds8 + k] = *(const uint4 *)(src + (size_t)sb * a.L1 + 8 * k);
-  }
+  fill_rows16(s2v, lds8, src, a.L1, 8, a.nsb, ext8);
   __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -370,28 +394,21 @@ __device__ __forceinline__ void stage2_u16v_body(const Stage2Args &a, uint32_t l
     const int m = j * a.ndm0 + mi;
     if (m >= a.ndm) break;
     const int32_t *__restrict__ rrow = a.res + (size_t)m * a.nsb;
+    // lane l sums outputs l, l+64, ...: every LDS read is an aligned 2-byte
+    // load, consecutive across the wave (a 16-byte read at the residual's
+    // arbitrary offset stalls the LDS on misalignment)
     uint32_t acc[P];
 #pragma unroll
     for (int i = 0; i < P; ++i) acc[i] = 0;
     for (int sb = 0; sb < a.nsb; ++sb) {
-      const uint16_t *p = s2 + sb * a.lds_len + lane * P + rrow[sb];
+      const uint16_t *p = s2 + sb * a.lds_len + lane + rrow[sb];
 #pragma unroll
-      for (int h = 0; h < P / 8; ++h) {
-        uint4 v;
-        __builtin_memcpy(&v, p + 8 * h, 16);
-        acc[8 * h + 0] = add_lo16(acc[8 * h + 0], v.x); acc[8 * h + 1] = add_hi16(acc[8 * h + 1], v.x);
-        acc[8 * h + 2] = add_lo16(acc[8 * h + 2], v.y); acc[8 * h + 3] = add_hi16(acc[8 * h + 3], v.y);
-        acc[8 * h + 4] = add_lo16(acc[8 * h + 4], v.z); acc[8 * h + 5] = add_hi16(acc[8 * h + 5], v.z);
-        acc[8 * h + 6] = add_lo16(acc[8 * h + 6], v.w); acc[8 * h + 7] = add_hi16(acc[8 * h + 7], v.w);
-      }
+      for (int i = 0; i < P; ++i) acc[i] += p[64 * i];
     }
-    uint32_t *o = (uint32_t *)a.out + (size_t)m * a.ostride + t0 + (size_t)lane * P;
-    if (t0 + (size_t)lane * P + P <= a.nout) {
-      __builtin_memcpy(o, acc, 4 * P);
-    } else {
-      for (int i = 0; i < P; ++i)
-        if (t0 + (size_t)lane * P + i < a.nout) o[i] = acc[i];
-    }
+    uint32_t *o = (uint32_t *)a.out + (size_t)m * a.ostride + t0 + (size_t)lane;
+#pragma unroll
+    for (int i = 0; i < P; ++i)
+      if (t0 + (size_t)lane + 64 * i < a.nout) o[64 * i] = acc[i];
   }
 }
 
@@ -427,10 +444,7 @@ __device__ __forceinline__ void stage2_u16c_body(const Stage2Args &a, uint32_t l
     for (int i = 0; i < P; ++i) acc[d][i] = 0;
   for (int sb0 = 0; sb0 < a.nsb; sb0 += a.sbc) {
     const int nsc = min(a.sbc, a.nsb - sb0);
-    for (int idx = threadIdx.x; idx < nsc * ext8; idx += blockDim.x) {
-      const int sb = idx / ext8, k = idx % ext8;
-      s2v[sb * lds8 + k] = *(const uint4 *)(src + (size_t)(sb0 + sb) * a.L1 + 8 * k);
-    }
+    fill_rows16(s2v, lds8, src + (size_t)sb0 * a.L1, a.L1, 8, nsc, ext8);
     __syncthreads();
 #pragma unroll
     for (int d = 0; d < MD; ++d) {
@@ -439,20 +453,9 @@ __device__ __forceinline__ void stage2_u16c_body(const Stage2Args &a, uint32_t l
       if (mi < a.ndm0 && m < a.ndm) {
         const int32_t *__restrict__ rrow = a.res + (size_t)m * a.nsb + sb0;
         for (int sb = 0; sb < nsc; ++sb) {
-          const uint16_t *p = s2 + sb * a.lds_len + lane * P + rrow[sb];
+          const uint16_t *p = s2 + sb * a.lds_len + lane + rrow[sb];
 #pragma unroll
-          for (int h = 0; h < P / 8; ++h) {
-            uint4 v;
-            __builtin_memcpy(&v, p + 8 * h, 16);
-            acc[d][8 * h + 0] = add_lo16(acc[d][8 * h + 0], v.x);
-            acc[d][8 * h + 1] = add_hi16(acc[d][8 * h + 1], v.x);
-            acc[d][8 * h + 2] = add_lo16(acc[d][8 * h + 2], v.y);
-            acc[d][8 * h + 3] = add_hi16(acc[d][8 * h + 3], v.y);
-            acc[d][8 * h + 4] = add_lo16(acc[d][8 * h + 4], v.z);
-            acc[d][8 * h + 5] = add_hi16(acc[d][8 * h + 5], v.z);
-            acc[d][8 * h + 6] = add_lo16(acc[d][8 * h + 6], v.w);
-            acc[d][8 * h + 7] = add_hi16(acc[d][8 * h + 7], v.w);
-          }
+          for (int i = 0; i < P; ++i) acc[d][i] += p[64 * i];
         }
       }
     }
@@ -463,13 +466,10 @@ __device__ __forceinline__ void stage2_u16c_body(const Stage2Args &a, uint32_t l
     const int mi = wave + d * nw;
     const int m = j * a.ndm0 + mi;
     if (mi >= a.ndm0 || m >= a.ndm) continue;
-    uint32_t *o = (uint32_t *)a.out + (size_t)m * a.ostride + t0 + (size_t)lane * P;
-    if (t0 + (size_t)lane * P + P <= a.nout) {
-      __builtin_memcpy(o, acc[d], 4 * P);
-    } else {
-      for (int i = 0; i < P; ++i)
-        if (t0 + (size_t)lane * P + i < a.nout) o[i] = acc[d][i];
-    }
+    uint32_t *o = (uint32_t *)a.out + (size_t)m * a.ostride + t0 + (size_t)lane;
+#pragma unroll
+    for (int i = 0; i < P; ++i)
+      if (t0 + (size_t)lane + 64 * i < a.nout) o[64 * i] = acc[d][i];
   }
 }
 
