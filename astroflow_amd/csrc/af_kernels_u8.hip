@@ -163,7 +163,9 @@ __device__ __forceinline__ void stats_u8_body(const StatArgs &sa, uint32_t bid, 
   }
 }
 
-__global__ __launch_bounds__(1024) void k_seg_stats_u8(StatArgs a) {
+// at most 64 VGPRs: two 1024-thread workgroups per CU (8 waves per SIMD)
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_seg_stats_u8(StatArgs a) {
   stats_u8_body(a, blockIdx.x, gridDim.x);
 }
 

@@ -390,7 +390,20 @@ static af_status build(af_plan *p) {
     p->wseg[2 * w] = (uint32_t)a;
     p->wseg[2 * w + 1] = (uint32_t)b + 1;
   }
-  const uint32_t stat_rows = d.dtype == AF_U8 ? kStatRowsU8 : kStatRows;
+  // u8 statistics: two 1024-thread workgroups fit a CU; rows per item are
+  // chosen so the items fill the machine in two equal rounds (a few hundred
+  // large items leave a short, mostly idle last round)
+  uint32_t stat_rows = d.dtype == AF_U8 ? kStatRowsU8 : kStatRows;
+  if (d.dtype == AF_U8) {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess ||
+        ncu <= 0)
+      ncu = 256;
+    const size_t slots = 2 * (size_t)ncu;
+    size_t r = (p->nbin + 2 * slots - 1) / (2 * slots);
+    r = (r + 3) / 4 * 4;
+    stat_rows = (uint32_t)std::min<size_t>(kStatRowsU8, std::max<size_t>(16, r));
+  }
   const uint32_t clean_rows =
       (d.dtype == AF_U8 && d.nchan <= kCleanU8MaxChan) ? kCleanRowsU8 : kCleanRows;
   p->seg_items.assign(1, 0);
