@@ -528,26 +528,43 @@ __device__ __forceinline__ uint4 realign_q(const uint4 lo, const uint4 hi, uint3
 }
 
 // Channels [i0, i1) of a row's sorted (channel, delay) list, all with delay
-// quarter class Q: two 16-byte aligned loads per lane (an aligned lane load
-// costs the L1 a fraction of a misaligned one) and a fixed realignment.
+// quarter class Q: one 16-byte aligned load per lane (an aligned lane load
+// costs the L1 a fraction of a misaligned one; each byte passes the L1 once),
+// the following 16 bytes from the neighbouring lane, and a fixed
+// realignment.
 template <int Q>
 __device__ __forceinline__ void dd_class(const uint8_t *__restrict__ tb, size_t Lp, uint32_t loff,
                                          const int2 *__restrict__ cd, int i0, int i1,
                                          uint32_t (&S)[4], uint32_t (&H)[4]) {
-#pragma unroll 8
-  for (int i = i0; i < i1; ++i) {
+  // (the DPP shifts are convergent, so the unrolling is written out: eight
+  // channels per step keep eight loads in flight, then a remainder loop)
+  auto body = [&](int i) {
     const int2 e = cd[i];
-    const uint8_t *rp = tb + (size_t)e.x * Lp + (e.y & ~15) + loff;
-    uint4 lo, hi;
-    __builtin_memcpy(&lo, __builtin_assume_aligned(rp, 16), 16);
-    __builtin_memcpy(&hi, __builtin_assume_aligned(rp + 16, 16), 16);
+    const uint8_t *rb = tb + (size_t)e.x * Lp + (e.y & ~15);   // wave-uniform
+    // one aligned 16-byte load per lane; the next 16 bytes are the right
+    // neighbour's (DPP wave shift), and lane 63's come from the one chunk
+    // past the wave's 1 KB, read at a uniform address
+    uint4 lo, nx;
+    __builtin_memcpy(&lo, __builtin_assume_aligned(rb + loff, 16), 16);
+    __builtin_memcpy(&nx, __builtin_assume_aligned(rb + kDdTile, 16), 16);
+    uint4 hi;
+    hi.x = (uint32_t)__builtin_amdgcn_update_dpp((int)nx.x, (int)lo.x, 0x130, 0xF, 0xF, false);
+    hi.y = (uint32_t)__builtin_amdgcn_update_dpp((int)nx.y, (int)lo.y, 0x130, 0xF, 0xF, false);
+    hi.z = (uint32_t)__builtin_amdgcn_update_dpp((int)nx.z, (int)lo.z, 0x130, 0xF, 0xF, false);
+    hi.w = (uint32_t)__builtin_amdgcn_update_dpp((int)nx.w, (int)lo.w, 0x130, 0xF, 0xF, false);
     const uint4 w = realign_q<Q>(lo, hi, (uint32_t)(e.y & 3));
     S[0] += w.x; S[1] += w.y; S[2] += w.z; S[3] += w.w;
     H[0] += odd_bytes(w.x);
     H[1] += odd_bytes(w.y);
     H[2] += odd_bytes(w.z);
     H[3] += odd_bytes(w.w);
+  };
+  int i = i0;
+  for (; i + 8 <= i1; i += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) body(i + u);
   }
+  for (; i < i1; ++i) body(i);
 }
 
 // u8 dedispersion over quarter-class-sorted channel lists (same tiling and
