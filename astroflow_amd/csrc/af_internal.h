@@ -35,6 +35,9 @@ struct Part {
 
 constexpr int kDdLanesOut = 16;               // outputs per lane, u8 dedispersion
 constexpr int kDdTile = 64 * kDdLanesOut;     // outputs per wave tile
+// The aligned-load u8 kernel: lane l + 1 hands lane l the 16 bytes after its
+// own, so lane 63 only loads and its outputs are not kept.
+constexpr int kDdTileQ = 63 * kDdLanesOut;
 constexpr int kDdRowsPerBlock = 4;            // DM rows (waves) per workgroup
 constexpr int kStatRows = 1024;               // rows per statistics item
 constexpr int kCleanRows = 64;                // rows per cleaning item
@@ -110,7 +113,7 @@ struct DdArgs {
   void *out;                // direct: [nrows][ostride]; grouped: [nrows][ngrp][ostride]
   size_t ostride;
   size_t nvalid;            // outputs written per row (direct: nout)
-  int ntiles;               // time tiles of kDdTile
+  int ntiles;               // time tiles of kDdTile (kDdTileQ: aligned-load u8 kernel)
   int out16;                // 1: u16 output (u8 input, group width <= 256)
   int lds_rows;             // u8 LDS-staged kernel: DM rows per workgroup (0 = off)
   int lds_win;              // bytes of a channel's delay window staged in LDS
@@ -136,6 +139,10 @@ struct Stage2Args {
   int lds_len;              // LDS row length (tile + max over groups of maxres)
   int inter16;              // inter is u16 (u8 input)
   int sbc;                  // u16 LDS kernel: subbands staged at a time (0 = all)
+  // All-subband u16 kernel: LDS row offsets [ndm_nom][nsb+1] in u16 elements.
+  // Row sb of group j holds the tile plus that subband's own largest residual
+  // over the group's fine DMs, rounded up to 8 elements; rows are packed.
+  const int32_t *sboff;
 };
 hipError_t launch_stage2(int dtype, const Stage2Args &a, hipStream_t st);
 // Single-DM spectrum: out[t][c] = xT[c][t0 + t + delay[c]], [tlen][nch].
@@ -201,7 +208,7 @@ enum LaunchKind : int {
   KB_CLEAN_U8, KB_CLEAN_U16, KB_CLEAN_U32, KB_CLEAN_F32,
   KB_DD_U8Q, KB_DD_U8, KB_DD_U8LDS, KB_DD_G_U8, KB_DD_G_U16, KB_DD_G_U32, KB_DD_G_F32F,
   KB_DD_G_F32D,
-  KB_S2_V8, KB_S2_V16, KB_S2_C8, KB_S2_C16, KB_S2_U16, KB_S2_G_U16, KB_S2_G_U32, KB_S2_G_U64, KB_S2_G_F32,
+  KB_S2_V4, KB_S2_V8, KB_S2_V16, KB_S2_C8, KB_S2_C16, KB_S2_U16, KB_S2_G_U16, KB_S2_G_U32, KB_S2_G_U64, KB_S2_G_F32,
   KB_S2_F32C8, KB_S2_F32C16,
   KB_COPY,
   KB_COUNT
@@ -263,7 +270,9 @@ struct af_plan {
   int nsuper = 1;           // super-tiles
   int inter_elem = 0;       // bytes per subband partial sum
   int s2_tile = 0, s2_lds = 0, s2_sbc = 0, maxlds = 0;
+  int s2_pk = 0;            // u16 elements of LDS per workgroup, packed rows (0: uniform rows)
   int dd_lds_rows = 0, dd_lds_win = 0;   // LDS-staged u8 dedispersion
+  bool dd_q = false;                     // aligned-load u8 dedispersion (kDdTileQ tiles)
   std::vector<int4> worklist;            // stage-1 workgroups in locality order
   std::vector<int2> chan_order;          // (channel, delay) sorted by delay quarter class
   std::vector<int> qstart;
@@ -271,7 +280,7 @@ struct af_plan {
   int *d_qstart = nullptr;
   int4 *d_worklist = nullptr;
   double adds = 0;
-  std::vector<int32_t> h_delays, h_res, h_maxres_j;
+  std::vector<int32_t> h_delays, h_res, h_maxres_j, h_maxres_jsb, h_sboff;
 
   // IQRM / segmentation
   int nwin = 0, nseg = 0, nlags = 0, Csub = 0;
@@ -282,7 +291,7 @@ struct af_plan {
   std::vector<int> seg_items, lags;
 
   // device buffers
-  int32_t *d_delays = nullptr, *d_res = nullptr, *d_maxres_j = nullptr;
+  int32_t *d_delays = nullptr, *d_res = nullptr, *d_maxres_j = nullptr, *d_sboff = nullptr;
   int *d_lags = nullptr, *d_seg_items = nullptr;
   uint32_t *d_wseg = nullptr, *d_wt0 = nullptr, *d_wt1 = nullptr;
   af::Seg *d_segs = nullptr;

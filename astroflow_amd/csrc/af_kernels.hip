@@ -531,7 +531,8 @@ __device__ __forceinline__ uint4 realign_q(const uint4 lo, const uint4 hi, uint3
 // quarter class Q: one 16-byte aligned load per lane (an aligned lane load
 // costs the L1 a fraction of a misaligned one; each byte passes the L1 once),
 // the following 16 bytes from the neighbouring lane, and a fixed
-// realignment.
+// realignment.  Lane 63 has no right neighbour: it only loads, for lane 62,
+// and its sums are discarded (a wave keeps 63 x 16 outputs).
 template <int Q>
 __device__ __forceinline__ void dd_class(const uint8_t *__restrict__ tb, size_t Lp, uint32_t loff,
                                          const int2 *__restrict__ cd, int i0, int i1,
@@ -542,16 +543,14 @@ __device__ __forceinline__ void dd_class(const uint8_t *__restrict__ tb, size_t 
     const int2 e = cd[i];
     const uint8_t *rb = tb + (size_t)e.x * Lp + (e.y & ~15);   // wave-uniform
     // one aligned 16-byte load per lane; the next 16 bytes are the right
-    // neighbour's (DPP wave shift), and lane 63's come from the one chunk
-    // past the wave's 1 KB, read at a uniform address
-    uint4 lo, nx;
+    // neighbour's (DPP wave shift left by one lane)
+    uint4 lo;
     __builtin_memcpy(&lo, __builtin_assume_aligned(rb + loff, 16), 16);
-    __builtin_memcpy(&nx, __builtin_assume_aligned(rb + kDdTile, 16), 16);
     uint4 hi;
-    hi.x = (uint32_t)__builtin_amdgcn_update_dpp((int)nx.x, (int)lo.x, 0x130, 0xF, 0xF, false);
-    hi.y = (uint32_t)__builtin_amdgcn_update_dpp((int)nx.y, (int)lo.y, 0x130, 0xF, 0xF, false);
-    hi.z = (uint32_t)__builtin_amdgcn_update_dpp((int)nx.z, (int)lo.z, 0x130, 0xF, 0xF, false);
-    hi.w = (uint32_t)__builtin_amdgcn_update_dpp((int)nx.w, (int)lo.w, 0x130, 0xF, 0xF, false);
+    hi.x = (uint32_t)__builtin_amdgcn_update_dpp((int)lo.x, (int)lo.x, 0x130, 0xF, 0xF, false);
+    hi.y = (uint32_t)__builtin_amdgcn_update_dpp((int)lo.y, (int)lo.y, 0x130, 0xF, 0xF, false);
+    hi.z = (uint32_t)__builtin_amdgcn_update_dpp((int)lo.z, (int)lo.z, 0x130, 0xF, 0xF, false);
+    hi.w = (uint32_t)__builtin_amdgcn_update_dpp((int)lo.w, (int)lo.w, 0x130, 0xF, 0xF, false);
     const uint4 w = realign_q<Q>(lo, hi, (uint32_t)(e.y & 3));
     S[0] += w.x; S[1] += w.y; S[2] += w.z; S[3] += w.w;
     H[0] += odd_bytes(w.x);
@@ -592,7 +591,7 @@ __device__ __forceinline__ void dd_u8_q_body(const DdArgs &a, uint32_t lid, uint
   const int cb = min(ca + a.grp_w, a.nch);
   const int blk_w = a.ngrp == 1 ? 256 : a.grp_w;
   const int nblk_all = (a.nch + blk_w - 1) / blk_w;
-  const uint8_t *__restrict__ tb = (const uint8_t *)a.xT + (size_t)tile * kDdTile;
+  const uint8_t *__restrict__ tb = (const uint8_t *)a.xT + (size_t)tile * kDdTileQ;
   const uint32_t loff = (uint32_t)lane * kDdLanesOut;
   const int2 *__restrict__ cd = a.chan_order + (size_t)row * a.nch;
   uint32_t tot[16];
@@ -617,7 +616,8 @@ __device__ __forceinline__ void dd_u8_q_body(const DdArgs &a, uint32_t lid, uint
       tot[4 * k + 3] += H[k] >> 16;
     }
   }
-  const size_t t = (size_t)tile * kDdTile + (size_t)lane * kDdLanesOut;
+  if (lane == 63) return;
+  const size_t t = (size_t)tile * kDdTileQ + (size_t)lane * kDdLanesOut;
   const size_t orow = ((size_t)row * a.ngrp + grp) * a.ostride;
   if (a.out16) {
     uint16_t *o = (uint16_t *)a.out + orow + t;
@@ -1223,8 +1223,8 @@ hipError_t launch_dedisperse(int dtype, const DdArgs &a, hipStream_t st) {
 }
 
 hipError_t launch_stage2(int dtype, const Stage2Args &a, hipStream_t st) {
-  if (dtype == AF_U8 && a.inter16 && (a.tile == 512 || a.tile == 1024))
-    return launch_stage2_u16v(a, st);   // all subbands in LDS, or in chunks
+  if (dtype == AF_U8 && a.inter16 && (a.sboff || a.tile == 512 || a.tile == 1024))
+    return launch_stage2_u16v(a, st);   // all subbands in LDS (packed rows), or in chunks
   if (dtype == AF_U8 && a.inter16 && a.tile > 0) {
     const int ntile = (int)((a.nout + a.tile - 1) / a.tile);
     const size_t lds = (size_t)a.nsb * a.lds_len * sizeof(uint16_t);

@@ -371,10 +371,11 @@ __global__ __launch_bounds__(1024) void k_clean_u8(CleanArgs a) {
 }
 
 // --------------------------------------------- subband stage 2 (u16 in) --
-// One workgroup (512 threads) per (coarse DM group, tile of 64*P outputs):
-// the group's partial sums for the tile plus its largest residual are staged
-// in LDS with 16-byte loads; each wave then sums the subbands of one fine DM
-// at a time, lane l of a wave summing outputs l, l+64, ..., l+64(P-1).
+// One workgroup (512 threads) per (coarse DM group j, tile of 64*P outputs).
+// Row sb of the group's partial sums is staged in LDS for the tile plus that
+// subband's own largest residual (rows packed at a.sboff[j][sb]); each wave
+// then sums the subbands of one fine DM at a time, lane l of a wave summing
+// outputs l, l+64, ..., l+64(P-1).
 template <int P>
 __device__ __forceinline__ void stage2_u16v_body(const Stage2Args &a, uint32_t lid, uint32_t) {
   extern __shared__ uint4 s2v[];
@@ -384,14 +385,47 @@ __device__ __forceinline__ void stage2_u16v_body(const Stage2Args &a, uint32_t l
   const int ti = (int)(lid % (uint32_t)ntile);
   const int j = (int)(lid / (uint32_t)ntile);
   const size_t t0 = (size_t)ti * TILE;
-  const int ext8 = (TILE + a.maxres_j[j] + 7) / 8;   // 16-byte chunks per subband row
-  const int lds8 = a.lds_len / 8;
+  const int32_t *__restrict__ off = a.sboff + (size_t)j * (a.nsb + 1);
   const uint16_t *__restrict__ src = (const uint16_t *)a.inter + (size_t)j * a.nsb * a.L1 + t0;
-  fill_rows16(s2v, lds8, src, a.L1, 8, a.nsb, ext8);
-  __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int nw = blockDim.x >> 6;
+  // fill: wave w stages rows w, w+nw, ...; a row of n 16-byte chunks takes
+  // ceil(n/64) wave-wide loads.  The (row, pass) walk is wave-uniform, and
+  // four loads are issued before their LDS stores.
+  {
+    int sb = wave, k0 = 0;
+    int o8 = sb < a.nsb ? off[sb] / 8 : 0;
+    int n8 = sb < a.nsb ? off[sb + 1] / 8 - o8 : 0;
+    while (sb < a.nsb) {
+      uint4 v[4];
+      int dst[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        dst[u] = -1;
+        if (sb < a.nsb) {
+          const int k = k0 + lane;
+          if (k < n8) {
+            v[u] = *(const uint4 *)(src + (size_t)sb * a.L1 + 8 * (size_t)k);
+            dst[u] = o8 + k;
+          }
+          k0 += 64;
+          if (k0 >= n8) {
+            sb += nw;
+            k0 = 0;
+            if (sb < a.nsb) {
+              o8 = off[sb] / 8;
+              n8 = off[sb + 1] / 8 - o8;
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (dst[u] >= 0) s2v[dst[u]] = v[u];
+    }
+  }
+  __syncthreads();
   for (int mi = wave; mi < a.ndm0; mi += nw) {
     const int m = j * a.ndm0 + mi;
     if (m >= a.ndm) break;
@@ -403,7 +437,7 @@ __device__ __forceinline__ void stage2_u16v_body(const Stage2Args &a, uint32_t l
 #pragma unroll
     for (int i = 0; i < P; ++i) acc[i] = 0;
     for (int sb = 0; sb < a.nsb; ++sb) {
-      const uint16_t *p = s2 + sb * a.lds_len + lane + rrow[sb];
+      const uint16_t *p = s2 + off[sb] + lane + rrow[sb];
 #pragma unroll
       for (int i = 0; i < P; ++i) acc[i] += p[64 * i];
     }
@@ -504,6 +538,7 @@ __device__ __forceinline__ int batch_find_u8(const uint32_t *__restrict__ start,
 AF_BATCHED_U8(kb_stat_u8, StatArgs, 1024, (stats_u8_body(a, lid, nb)))
 AF_BATCHED_U8(kb_red_u8, ReduceArgs, 1024, (reduce_u8_body(a, lid, nb)))
 AF_BATCHED_U8(kb_clean_u8, CleanArgs, 1024, (clean_u8_body(a, lid, nb)))
+AF_BATCHED_U8(kb_s2_v4, Stage2Args, 512, (stage2_u16v_body<4>(a, lid, nb)))
 AF_BATCHED_U8(kb_s2_v8, Stage2Args, 512, (stage2_u16v_body<8>(a, lid, nb)))
 AF_BATCHED_U8(kb_s2_v16, Stage2Args, 512, (stage2_u16v_body<16>(a, lid, nb)))
 AF_BATCHED_U8(kb_s2_c8, Stage2Args, 512, (stage2_u16c_body<8>(a, lid, nb)))
@@ -632,16 +667,17 @@ hipError_t launch_clean_u8(const CleanArgs &a, hipStream_t st) {
 
 hipError_t launch_stage2_u16v(const Stage2Args &a, hipStream_t st) {
   const int P = a.tile / 64;
-  if (P != 8 && P != 16) return hipErrorInvalidValue;
   const int ntile = (int)((a.nout + a.tile - 1) / a.tile);
   const size_t nblk = (size_t)ntile * a.ndm_nom;
   if (a.sbc > 0 && a.sbc < a.nsb) {
-    if (a.ndm0 > 32) return hipErrorInvalidValue;
+    if (a.ndm0 > 32 || (P != 8 && P != 16)) return hipErrorInvalidValue;
     const size_t lds = (size_t)a.sbc * a.lds_len * sizeof(uint16_t);
     return emit_args(P == 16 ? KB_S2_C16 : KB_S2_C8, 8, nblk, 512, lds, a, st);
   }
-  const size_t lds = (size_t)a.nsb * a.lds_len * sizeof(uint16_t);
-  return emit_args(P == 16 ? KB_S2_V16 : KB_S2_V8, 8, nblk, 512, lds, a, st);
+  // all subbands: packed rows, a.lds_len u16 elements per workgroup
+  if (!a.sboff || (P != 4 && P != 8 && P != 16)) return hipErrorInvalidValue;
+  const size_t lds = (size_t)a.lds_len * sizeof(uint16_t);
+  return emit_args(P == 16 ? KB_S2_V16 : P == 8 ? KB_S2_V8 : KB_S2_V4, 8, nblk, 512, lds, a, st);
 }
 
 template <typename A>
@@ -655,6 +691,11 @@ hipError_t launch_desc_u8(const LaunchDesc &d, hipStream_t st) {
   case KB_STAT_U8: hipLaunchKernelGGL(k_seg_stats_u8, g, b, d.lds, st, argsof_u8<StatArgs>(d)); break;
   case KB_RED_U8: hipLaunchKernelGGL(k_seg_reduce_u8, g, b, d.lds, st, argsof_u8<ReduceArgs>(d)); break;
   case KB_CLEAN_U8: hipLaunchKernelGGL(k_clean_u8, g, b, d.lds, st, argsof_u8<CleanArgs>(d)); break;
+  case KB_S2_V4:
+    (void)hipFuncSetAttribute((const void *)k_stage2_u16v<4>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)d.lds);
+    hipLaunchKernelGGL(k_stage2_u16v<4>, g, b, d.lds, st, argsof_u8<Stage2Args>(d));
+    break;
   case KB_S2_V8:
     (void)hipFuncSetAttribute((const void *)k_stage2_u16v<8>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)d.lds);
@@ -693,6 +734,11 @@ hipError_t launch_batched_u8(int kind, const void *tab, const uint32_t *start, i
     break;
   case KB_CLEAN_U8:
     hipLaunchKernelGGL(kb_clean_u8, g, dim3(1024), lds, st, (const CleanArgs *)tab, start, n);
+    break;
+  case KB_S2_V4:
+    (void)hipFuncSetAttribute((const void *)kb_s2_v4, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    hipLaunchKernelGGL(kb_s2_v4, g, dim3(512), lds, st, (const Stage2Args *)tab, start, n);
     break;
   case KB_S2_V8:
     (void)hipFuncSetAttribute((const void *)kb_s2_v8, hipFuncAttributeMaxDynamicSharedMemorySize,

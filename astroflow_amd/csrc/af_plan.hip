@@ -203,6 +203,7 @@ static af_status build(af_plan *p) {
     }
     p->h_res.resize((size_t)p->ndm * p->nsb);
     p->h_maxres_j.assign(p->ndm_nom, 0);
+    p->h_maxres_jsb.assign((size_t)p->ndm_nom * p->nsb, 0);
     long maxr = 0;
     for (int m = 0; m < p->ndm; ++m) {
       const int j = m / ndm0;
@@ -215,6 +216,8 @@ static af_status build(af_plan *p) {
         if (r > 0x3FFFFFFF) return fail(AF_ERR_INVALID, "residual too large");
         p->h_res[(size_t)m * p->nsb + sb] = (int32_t)r;
         p->h_maxres_j[j] = std::max<int32_t>(p->h_maxres_j[j], (int32_t)r);
+        int32_t &mj = p->h_maxres_jsb[(size_t)j * p->nsb + sb];
+        mj = std::max<int32_t>(mj, (int32_t)r);
         maxr = std::max(maxr, r);
       }
     }
@@ -243,17 +246,41 @@ static af_status build(af_plan *p) {
     int tile = 0;
     if (inter16) {
       auto lds_of = [&](int t) { return (size_t)p->nsb * (size_t)((t + p->maxres + 7) / 8 * 8) * 2; };
+      // all-subband kernel: each subband's row holds the tile plus its own
+      // largest residual, so rows are packed (bytes of the largest group)
+      auto lds_pk = [&](int t) {
+        size_t best = 0;
+        for (int j = 0; j < p->ndm_nom; ++j) {
+          size_t e = 0;
+          for (int sb = 0; sb < p->nsb; ++sb)
+            e += (size_t)((t + p->h_maxres_jsb[(size_t)j * p->nsb + sb] + 7) / 8 * 8);
+          best = std::max(best, e);
+        }
+        return best * 2;
+      };
+      bool packed = false;
       // a requested stage-2 tile (autotuner), when it fits
-      if ((d.tile_t == 1024 || d.tile_t == 512) && lds_of(d.tile_t) <= (size_t)maxlds)
+      if ((d.tile_t == 1024 || d.tile_t == 512 || d.tile_t == 256) &&
+          lds_pk(d.tile_t) <= (size_t)maxlds) {
         tile = d.tile_t;
-      if ((d.tile_t == 256 || d.tile_t == 128 || d.tile_t == 64) &&
-          lds_of(d.tile_t) <= (size_t)std::min(maxlds, 65536))
+        packed = true;
+      }
+      if ((d.tile_t == 128 || d.tile_t == 64) && lds_of(d.tile_t) <= (size_t)std::min(maxlds, 65536))
         tile = d.tile_t;
-      // 16-byte-load kernel: prefer two workgroups per CU, then one
+      // all-subband kernel: prefer two workgroups per CU, then one
       const int big[] = {1024, 512};
       for (int half = 2; half >= 1 && !tile; --half)
         for (int t : big)
-          if (lds_of(t) <= (size_t)maxlds / half) { tile = t; break; }
+          if (lds_pk(t) <= (size_t)maxlds / half) { tile = t; packed = true; break; }
+      if (packed) {
+        p->h_sboff.assign((size_t)p->ndm_nom * (p->nsb + 1), 0);
+        for (int j = 0; j < p->ndm_nom; ++j) {
+          int32_t *o = &p->h_sboff[(size_t)j * (p->nsb + 1)];
+          for (int sb = 0; sb < p->nsb; ++sb)
+            o[sb + 1] = o[sb] + (tile + p->h_maxres_jsb[(size_t)j * p->nsb + sb] + 7) / 8 * 8;
+        }
+        p->s2_pk = (int)(lds_pk(tile) / 2);
+      }
       // many subbands: stage them in chunks (<= 64 KB per workgroup)
       if (!tile && d.ndm0 <= 32) {
         for (int t : big) {
@@ -326,8 +353,12 @@ static af_status build(af_plan *p) {
       p->dd_lds_win = win;
     }
   }
-  p->ntiles1 = (int)((n_ext + kDdTile - 1) / kDdTile);
-  p->L1 = (size_t)p->ntiles1 * kDdTile;
+  // the aligned-load u8 kernel (channel lists sorted by delay quarter class)
+  p->dd_q = d.dtype == AF_U8 && p->dd_lds_rows == 0 &&
+            (d.method == AF_METHOD_DIRECT || d.sb_nchan <= 256);
+  const int dd_tile = p->dd_q ? kDdTileQ : kDdTile;
+  p->ntiles1 = (int)((n_ext + dd_tile - 1) / dd_tile);
+  p->L1 = (size_t)p->ntiles1 * dd_tile;
   // Row length of the channel-major buffer: every load of the dedispersion
   // kernels (tile start + delay + 16 bytes, or + the LDS window) stays inside
   // its own channel row, which is zero beyond nbin.
@@ -433,7 +464,8 @@ static af_status build(af_plan *p) {
       const int cmid = std::min(p->nch - 1, g * gw + (std::min(gw, p->nch - g * gw) - 1) / 2);
       for (int t = 0; t < p->ntiles1; ++t)
         for (int b = 0; b < nrb; ++b) {
-          const int64_t start = (int64_t)t * kDdTile + p->h_delays[(size_t)(b * JB) * p->nch + cmid];
+          const int64_t start = (int64_t)t * (p->dd_q ? kDdTileQ : kDdTile) +
+                                p->h_delays[(size_t)(b * JB) * p->nch + cmid];
           keyed.push_back({(int64_t)g << 40 | start, int4{g, t, b, 0}});
         }
     }
@@ -446,8 +478,7 @@ static af_status build(af_plan *p) {
   // ---- channel order for the aligned-load u8 kernel: per DM row and channel
   // block (<= 256 channels), channels sorted by the quarter (delay mod 16)/4
   // of their delay, with the start of each class.
-  if (d.dtype == AF_U8 && p->dd_lds_rows == 0 &&
-      (d.method == AF_METHOD_DIRECT || d.sb_nchan <= 256)) {
+  if (p->dd_q) {
     const int rows = d.method == AF_METHOD_DIRECT ? p->ndm : p->ndm_nom;
     const int bw = d.method == AF_METHOD_DIRECT ? 256 : d.sb_nchan;
     const int nblk = (p->nch + bw - 1) / bw;
@@ -475,6 +506,7 @@ static af_status build(af_plan *p) {
   AF_TRY(upload(p, &p->d_delays, p->h_delays));
   AF_TRY(upload(p, &p->d_res, p->h_res));
   AF_TRY(upload(p, &p->d_maxres_j, p->h_maxres_j));
+  if (!p->h_sboff.empty()) AF_TRY(upload(p, &p->d_sboff, p->h_sboff));
   AF_TRY(upload(p, &p->d_lags, p->lags));
   AF_TRY(upload(p, &p->d_seg_items, p->seg_items));
   AF_TRY(upload(p, &p->d_wseg, p->wseg));
@@ -649,9 +681,10 @@ af_status run_dedisp(af_plan *p, const void *xT, size_t Lp, void *d_out, hipStre
     sa.ndm_nom = p->ndm_nom;
     sa.ostride = p->nout;
     sa.tile = p->s2_tile;
-    sa.lds_len = p->s2_lds;
+    sa.lds_len = p->s2_pk ? p->s2_pk : p->s2_lds;   // packed rows: elements per workgroup
     sa.inter16 = inter16 ? 1 : 0;
     sa.sbc = p->s2_sbc;
+    sa.sboff = p->d_sboff;
     // super-tile k: stage 1 reads the cleaned buffer from column k*T2,
     // stage 2 writes output columns [k*T2, min(nout, (k+1)*T2))
     for (int k = 0; k < p->nsuper; ++k) {

@@ -177,15 +177,17 @@ def test_subband_time_super_tiles(gpu_lib, dtype, tile):
         assert np.array_equal(got, ref)
 
 
-@pytest.mark.parametrize("tile", [0, 512])
-def test_subband_u8_many_subbands_chunked_stage2(gpu_lib, tile):
-    # 64 subbands with a ~1400-sample residual halo do not fit LDS at once:
-    # stage 2 stages the subbands in chunks (13 at tile 1024, 16 at tile 512)
-    say(f"case: subband u8 2048ch, 32-ch subbands, large halo (stage-2 tile {tile})")
+@pytest.mark.parametrize("sbw,tile", [(16, 0), (16, 512), (32, 0), (32, 1024), (32, 256)])
+def test_subband_u8_many_subbands_large_halo(gpu_lib, sbw, tile):
+    # A large residual halo (~1400 samples at the lowest subband, a few at the
+    # highest).  64 subbands fit LDS as packed rows, each row holding the tile
+    # plus its own subband's halo; 128 subbands do not, and stage 2 stages them
+    # in chunks.
+    say(f"case: subband u8 2048ch, {sbw}-ch subbands, large halo (stage-2 tile {tile})")
     rng = np.random.default_rng(23)
     b = synth.band_1000_1500(2048)
     x = noise(rng, 6000, 2048)
-    spec = make_spec(x, b, 100.0, 2.0, 64, method="subband", sbw=32, ndm0=16, tile_t=tile,
+    spec = make_spec(x, b, 100.0, 2.0, 64, method="subband", sbw=sbw, ndm0=16, tile_t=tile,
                      use_zero_dm=True, zdm_c0=0, zdm_c1=2048)
     got, ref, *_ = run_both(x, spec, want_masks=False)
     assert np.array_equal(got, ref)
