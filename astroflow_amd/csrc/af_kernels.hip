@@ -533,50 +533,88 @@ __device__ __forceinline__ uint4 realign_q(const uint4 lo, const uint4 hi, uint3
 // the following 16 bytes from the neighbouring lane, and a fixed
 // realignment.  Lane 63 has no right neighbour: it only loads, for lane 62,
 // and its sums are discarded (a wave keeps 63 x 16 outputs).
-template <int Q>
+template <int Q, bool O32>
 __device__ __forceinline__ void dd_class(const uint8_t *__restrict__ tb, size_t Lp, uint32_t loff,
                                          const int2 *__restrict__ cd, int i0, int i1,
                                          uint32_t (&S)[4], uint32_t (&H)[4]) {
-  // (the DPP shifts are convergent, so the unrolling is written out: eight
-  // channels per step keep eight loads in flight, then a remainder loop)
-  auto body = [&](int i) {
-    const int2 e = cd[i];
-    const uint8_t *rb = tb + (size_t)e.x * Lp + (e.y & ~15);   // wave-uniform
-    // one aligned 16-byte load per lane; the next 16 bytes are the right
-    // neighbour's (DPP wave shift left by one lane)
-    uint4 lo;
-    __builtin_memcpy(&lo, __builtin_assume_aligned(rb + loff, 16), 16);
+  // Row start of a (channel, delay) entry, wave-uniform: the channel's row
+  // plus the delay rounded down to 16 bytes.  O32: the buffer is below 4 GiB,
+  // so the offset is formed in 32 bits.
+  // O32: a raw buffer load, base tb, lane offset in a VGPR and the row
+  // offset in an SGPR (no 64-bit address arithmetic per channel)
+  const __amdgpu_buffer_rsrc_t rsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void *)tb, (short)0, -1, 0x00020000);
+  auto load = [&](const int2 e, uint4 &lo) {
+    if (O32) {
+      const uint32_t off = (uint32_t)e.x * (uint32_t)Lp + ((uint32_t)e.y & ~15u);
+      const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)loff, (int)off, 0);
+      lo = make_uint4(v[0], v[1], v[2], v[3]);
+    } else {
+      const uint8_t *rb = tb + (size_t)e.x * Lp + (e.y & ~15);
+      __builtin_memcpy(&lo, __builtin_assume_aligned(rb + loff, 16), 16);
+    }
+  };
+  // the next 16 bytes after a lane's chunk are the right neighbour's chunk
+  // (DPP wave shift left by one lane), then a fixed realignment
+  auto add = [&](const uint4 lo, const uint32_t r) {
+    // (bound_ctrl: lane 63, whose sums are discarded, reads zero)
     uint4 hi;
-    hi.x = (uint32_t)__builtin_amdgcn_update_dpp((int)lo.x, (int)lo.x, 0x130, 0xF, 0xF, false);
-    hi.y = (uint32_t)__builtin_amdgcn_update_dpp((int)lo.y, (int)lo.y, 0x130, 0xF, 0xF, false);
-    hi.z = (uint32_t)__builtin_amdgcn_update_dpp((int)lo.z, (int)lo.z, 0x130, 0xF, 0xF, false);
-    hi.w = (uint32_t)__builtin_amdgcn_update_dpp((int)lo.w, (int)lo.w, 0x130, 0xF, 0xF, false);
-    const uint4 w = realign_q<Q>(lo, hi, (uint32_t)(e.y & 3));
+    hi.x = (uint32_t)__builtin_amdgcn_mov_dpp((int)lo.x, 0x130, 0xF, 0xF, true);
+    hi.y = (uint32_t)__builtin_amdgcn_mov_dpp((int)lo.y, 0x130, 0xF, 0xF, true);
+    hi.z = (uint32_t)__builtin_amdgcn_mov_dpp((int)lo.z, 0x130, 0xF, 0xF, true);
+    hi.w = (uint32_t)__builtin_amdgcn_mov_dpp((int)lo.w, 0x130, 0xF, 0xF, true);
+    const uint4 w = realign_q<Q>(lo, hi, r);
     S[0] += w.x; S[1] += w.y; S[2] += w.z; S[3] += w.w;
     H[0] += odd_bytes(w.x);
     H[1] += odd_bytes(w.y);
     H[2] += odd_bytes(w.z);
     H[3] += odd_bytes(w.w);
   };
+  // eight channels per step: the eight loads are issued before any is used
+  // (the DPP moves are convergent, so the unrolling is written out), then a
+  // remainder loop
+  // the (channel, delay) list is read-only for the kernel's lifetime:
+  // constant address space, so the wave-uniform reads are scalar loads
+  typedef __attribute__((address_space(4))) const uint64_t cu64;
+  cu64 *cdc = (cu64 *)cd;
+  auto entry = [&](int k) {
+    const uint64_t v = cdc[k];
+    return make_int2((int)(uint32_t)v, (int)(uint32_t)(v >> 32));
+  };
   int i = i0;
   for (; i + 8 <= i1; i += 8) {
+    uint4 lo[8];
+    uint32_t r[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) body(i + u);
+    for (int u = 0; u < 8; ++u) {
+      const int2 e = entry(i + u);
+      load(e, lo[u]);
+      r[u] = (uint32_t)e.y & 3u;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) add(lo[u], r[u]);
   }
-  for (; i < i1; ++i) body(i);
+  for (; i < i1; ++i) {
+    const int2 e = entry(i);
+    uint4 lo;
+    load(e, lo);
+    add(lo, (uint32_t)e.y & 3u);
+  }
 }
 
 // u8 dedispersion over quarter-class-sorted channel lists (same tiling and
 // packed-lane accumulation as k_dd_u8; channel blocks of <= 256 channels,
 // each summed in the four classes and flushed).
+template <bool O32>
 __device__ __forceinline__ void dd_u8_q_body(const DdArgs &a, uint32_t lid, uint32_t) {
   const int nrowblk = (a.nrows + kDdRowsPerBlock - 1) / kDdRowsPerBlock;
   int rowblk, tile, grp;
   if (a.worklist) {
     const int4 wi = a.worklist[lid];
-    grp = wi.x;
-    tile = wi.y;
-    rowblk = wi.z;
+    // workgroup-uniform: in SGPRs (the buffer descriptor is built from them)
+    grp = __builtin_amdgcn_readfirstlane(wi.x);
+    tile = __builtin_amdgcn_readfirstlane(wi.y);
+    rowblk = __builtin_amdgcn_readfirstlane(wi.z);
   } else {
     rowblk = (int)(lid % (uint32_t)nrowblk);
     const uint32_t rest = lid / (uint32_t)nrowblk;
@@ -603,10 +641,10 @@ __device__ __forceinline__ void dd_u8_q_body(const DdArgs &a, uint32_t lid, uint
     const int *qs = a.qstart + ((size_t)row * nblk_all + blk) * 4;
     const int q0 = qs[0], q1 = qs[1], q2 = qs[2], q3 = qs[3];
     uint32_t S[4] = {0, 0, 0, 0}, H[4] = {0, 0, 0, 0};
-    dd_class<0>(tb, a.Lp, loff, cd, q0, q1, S, H);
-    dd_class<1>(tb, a.Lp, loff, cd, q1, q2, S, H);
-    dd_class<2>(tb, a.Lp, loff, cd, q2, q3, S, H);
-    dd_class<3>(tb, a.Lp, loff, cd, q3, bend, S, H);
+    dd_class<0, O32>(tb, a.Lp, loff, cd, q0, q1, S, H);
+    dd_class<1, O32>(tb, a.Lp, loff, cd, q1, q2, S, H);
+    dd_class<2, O32>(tb, a.Lp, loff, cd, q2, q3, S, H);
+    dd_class<3, O32>(tb, a.Lp, loff, cd, q3, bend, S, H);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const uint32_t L = S[k] - (H[k] << 8);
@@ -644,7 +682,9 @@ __device__ __forceinline__ void dd_u8_q_body(const DdArgs &a, uint32_t lid, uint
 }
 
 __global__ __launch_bounds__(256) void k_dd_u8_q(DdArgs a) {
-  dd_u8_q_body(a, xcd_remap(blockIdx.x, gridDim.x), gridDim.x);
+  const uint32_t lid = xcd_remap(blockIdx.x, gridDim.x);
+  if ((uint64_t)a.nch * a.Lp <= 0xFFFFFFFFull) dd_u8_q_body<true>(a, lid, gridDim.x);
+  else dd_u8_q_body<false>(a, lid, gridDim.x);
 }
 
 __device__ __forceinline__ void dd_u8_body(const DdArgs &a, uint32_t lid, uint32_t) {
@@ -1093,7 +1133,20 @@ AF_BATCHED(kb_segmask, SegmaskArgs, 256, (segmask_body(a, lid, nb)))
 AF_BATCHED(kb_clean_u16, CleanArgs, 256, (clean_body<uint16_t>(a, lid, nb)))
 AF_BATCHED(kb_clean_u32, CleanArgs, 256, (clean_body<uint32_t>(a, lid, nb)))
 AF_BATCHED(kb_clean_f32, CleanArgs, 256, (clean_body<float>(a, lid, nb)))
-AF_BATCHED(kb_dd_u8q, DdArgs, 256, (dd_u8_q_body(a, lid, nb)))
+#define AF_BATCHED_REF(NAME, ARGS, BOUNDS, CALL)                                       \
+  __global__ __launch_bounds__(BOUNDS) void NAME(                                         \
+      const ARGS *__restrict__ tab, const uint32_t *__restrict__ start, int n) {         \
+    const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);                                \
+    const int i = batch_find(start, n, g);                                              \
+    const ARGS &a = tab[i];                                                             \
+    const uint32_t lid = g - start[i], nb = start[i + 1] - start[i];                    \
+    (void)nb;                                                                           \
+    CALL;                                                                               \
+  }
+
+AF_BATCHED_REF(kb_dd_u8q, DdArgs, 256,
+                ((uint64_t)a.nch * a.Lp <= 0xFFFFFFFFull ? dd_u8_q_body<true>(a, lid, nb)
+                                                         : dd_u8_q_body<false>(a, lid, nb)))
 AF_BATCHED(kb_dd_u8, DdArgs, 256, (dd_u8_body(a, lid, nb)))
 AF_BATCHED(kb_dd_g_u8, DdArgs, 256, (dd_generic_body<uint8_t, uint32_t, uint32_t>(a, lid, nb)))
 AF_BATCHED(kb_dd_g_u16, DdArgs, 256, (dd_generic_body<uint16_t, uint32_t, uint32_t>(a, lid, nb)))

@@ -118,18 +118,22 @@ __device__ __forceinline__ void stats_u8_body(const StatArgs &sa, uint32_t bid, 
   if (active) {
     const uint8_t *row = x + (size_t)(it.r0 + rl) * nchan;
     const size_t step = (size_t)4 * nchan;
-    if (cb + 16 <= c1) {
-      // whole 16-channel groups: straight-line loads, several in flight
+    if (cb + 16 <= nchan) {
+      // the 16-byte load stays inside the row: straight-line loads, several
+      // in flight, one code path for the whole wave.  Channels at or beyond
+      // c1 (a group that straddles the range end) are summed in their own
+      // byte lanes and never written.
+      auto ldm = [&](const uint8_t *p) { return ld16(p); };
       uint32_t r = it.r0 + rl;
       for (; r + 12 < it.r1; r += 16, row += 4 * step) {
-        const uint4 w0 = ld16(row + cb), w1 = ld16(row + step + cb);
-        const uint4 w2 = ld16(row + 2 * step + cb), w3 = ld16(row + 3 * step + cb);
+        const uint4 w0 = ldm(row + cb), w1 = ldm(row + step + cb);
+        const uint4 w2 = ldm(row + 2 * step + cb), w3 = ldm(row + 3 * step + cb);
         accum(w0);
         accum(w1);
         accum(w2);
         accum(w3);
       }
-      for (; r < it.r1; r += 4, row += step) accum(ld16(row + cb));
+      for (; r < it.r1; r += 4, row += step) accum(ldm(row + cb));
     } else {
       for (uint32_t r = it.r0 + rl; r < it.r1; r += 4, row += step) accum(load_row16(row, cb, c1));
     }
