@@ -294,12 +294,26 @@ __device__ __forceinline__ void lag_body(const LagArgs &a, uint32_t bid, uint32_
     srt[k] = dv;
   }
   __syncthreads();
-  // bitonic sort, ascending
+  // bitonic sort, ascending.  Thread t takes the pairs p = t + m*blockDim.
+  // A pass with distance j <= 64 keeps each wave inside its own 128-element
+  // blocks, so two such passes in a row need only the wave's own ordering;
+  // any pass next to a longer one is separated by a workgroup barrier.
   const int half = P2 >> 1;
+  bool prev_local = false;
   for (int kk = 2; kk <= P2; kk <<= 1) {
     for (int j = kk >> 1; j > 0; j >>= 1) {
+      const bool local = j <= 64;
+      if (local && prev_local) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      } else {
+        __syncthreads();
+      }
+      prev_local = local;
+      const int lj = __builtin_ctz((unsigned)j);
       for (int p = threadIdx.x; p < half; p += blockDim.x) {
-        const int i = 2 * j * (p / j) + (p % j);
+        const int i = ((p >> lj) << (lj + 1)) | (p & (j - 1));
         const int q = i + j;
         const bool asc = (i & kk) == 0;
         float a = srt[i], b = srt[q];
@@ -308,9 +322,9 @@ __device__ __forceinline__ void lag_body(const LagArgs &a, uint32_t bid, uint32_
           srt[q] = a;
         }
       }
-      __syncthreads();
     }
   }
+  __syncthreads();
   if (threadIdx.x == 0) {
     float qv[3];
     const float qs[3] = {0.25f, 0.50f, 0.75f};
@@ -350,20 +364,43 @@ __device__ __forceinline__ int iqrm_partner(int k, int lag, int C) {
   return (k < lag) ? 0 : k - lag;
 }
 
-// Votes and final window mask (src/core/iqrm.cu:138-185).
+// Votes and final window mask (src/core/iqrm.cu:138-185).  The window's
+// flags are first gathered into one bit per lag per channel in LDS (eight
+// byte loads in flight per thread; nw = ceil(nlags / 32) words per channel),
+// then the casting counts and the test read only LDS.
+// LDS: Csub x (1 + nw) x 4 bytes.
 __device__ __forceinline__ void merge_body(const MergeArgs &a, uint32_t bid, uint32_t) {
   const uint8_t *__restrict__ flags = a.flags;
   const int *__restrict__ lags = a.lags;
   const int nlags = a.nlags, Csub = a.Csub, c0 = a.c0, nchan = a.nchan;
+  const int nw = (nlags + 31) / 32;
   uint8_t *__restrict__ masks = a.masks;
   extern __shared__ int cast[];
+  uint32_t *bits = (uint32_t *)(cast + Csub);   // [nw][Csub]
   const int w = (int)bid;
   const uint8_t *f = flags + (size_t)w * nlags * Csub;
-  for (int k = threadIdx.x; k < Csub; k += blockDim.x) cast[k] = 0;
+  for (int k = threadIdx.x; k < Csub; k += blockDim.x) {
+    for (int l0 = 0; l0 < nlags; l0 += 32) {
+      const int l1 = min(l0 + 32, nlags);
+      uint32_t b = 0;
+      int l = l0;
+      for (; l + 8 <= l1; l += 8) {
+        uint8_t v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = f[(size_t)(l + u) * Csub + k];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) b |= (v[u] ? 1u : 0u) << (l + u - l0);
+      }
+      for (; l < l1; ++l) b |= (f[(size_t)l * Csub + k] ? 1u : 0u) << (l - l0);
+      bits[(size_t)(l0 >> 5) * Csub + k] = b;
+    }
+    cast[k] = 0;
+  }
   __syncthreads();
   for (int k = threadIdx.x; k < Csub; k += blockDim.x)
     for (int l = 0; l < nlags; ++l)
-      if (f[(size_t)l * Csub + k]) atomicAdd(&cast[iqrm_partner(k, lags[l], Csub)], 1);
+      if ((bits[(size_t)(l >> 5) * Csub + k] >> (l & 31)) & 1u)
+        atomicAdd(&cast[iqrm_partner(k, lags[l], Csub)], 1);
   __syncthreads();
   uint8_t *mrow = masks + (size_t)w * nchan;
   for (int c = threadIdx.x; c < nchan; c += blockDim.x) {
@@ -371,10 +408,11 @@ __device__ __forceinline__ void merge_body(const MergeArgs &a, uint32_t bid, uin
     uint8_t m = 0;
     if (k >= 0 && k < Csub) {
       int recv = 0;
-      for (int l = 0; l < nlags; ++l) recv += f[(size_t)l * Csub + k];
+      for (int q = 0; q < nw; ++q) recv += __builtin_popcount(bits[(size_t)q * Csub + k]);
       if (recv > 0)
         for (int l = 0; l < nlags; ++l)
-          if (f[(size_t)l * Csub + k] && cast[iqrm_partner(k, lags[l], Csub)] < recv) {
+          if (((bits[(size_t)(l >> 5) * Csub + k] >> (l & 31)) & 1u) &&
+              cast[iqrm_partner(k, lags[l], Csub)] < recv) {
             m = 1;
             break;
           }
@@ -1237,7 +1275,8 @@ hipError_t launch_iqrm_lags(int dtype, const Part *segsum, const Seg *segs,
 hipError_t launch_iqrm_merge(const uint8_t *flags, const int *lags, int nlags, int nwin,
                              int Csub, int c0, int nchan, uint8_t *masks, hipStream_t st) {
   MergeArgs a{flags, lags, masks, nlags, Csub, c0, nchan};
-  return emit(KB_MERGE, 4, (size_t)nwin, 1024, (size_t)Csub * sizeof(int), a, st);
+  const size_t lds = (size_t)Csub * (1 + (nlags + 31) / 32) * sizeof(int);
+  return emit(KB_MERGE, 4, (size_t)nwin, 1024, lds, a, st);
 }
 
 hipError_t launch_segmask(const uint8_t *masks, const Seg *segs, int nseg, int nchan,
@@ -1342,11 +1381,36 @@ hipError_t launch_desc(const LaunchDesc &d, hipStream_t st) {
   case KB_RED_U16: hipLaunchKernelGGL(k_seg_reduce<uint16_t>, g, b, d.lds, st, argsof<ReduceArgs>(d)); break;
   case KB_RED_U32: hipLaunchKernelGGL(k_seg_reduce<uint32_t>, g, b, d.lds, st, argsof<ReduceArgs>(d)); break;
   case KB_RED_F32: hipLaunchKernelGGL(k_seg_reduce<float>, g, b, d.lds, st, argsof<ReduceArgs>(d)); break;
-  case KB_LAG_U8: hipLaunchKernelGGL(k_iqrm_lag<uint8_t>, g, b, d.lds, st, argsof<LagArgs>(d)); break;
-  case KB_LAG_U16: hipLaunchKernelGGL(k_iqrm_lag<uint16_t>, g, b, d.lds, st, argsof<LagArgs>(d)); break;
-  case KB_LAG_U32: hipLaunchKernelGGL(k_iqrm_lag<uint32_t>, g, b, d.lds, st, argsof<LagArgs>(d)); break;
-  case KB_LAG_F32: hipLaunchKernelGGL(k_iqrm_lag<float>, g, b, d.lds, st, argsof<LagArgs>(d)); break;
-  case KB_MERGE: hipLaunchKernelGGL(k_iqrm_merge, g, b, d.lds, st, argsof<MergeArgs>(d)); break;
+  case KB_LAG_U8:
+    if (d.lds > 65536)
+      (void)hipFuncSetAttribute((const void *)k_iqrm_lag<uint8_t>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)d.lds);
+    hipLaunchKernelGGL(k_iqrm_lag<uint8_t>, g, b, d.lds, st, argsof<LagArgs>(d));
+    break;
+  case KB_LAG_U16:
+    if (d.lds > 65536)
+      (void)hipFuncSetAttribute((const void *)k_iqrm_lag<uint16_t>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)d.lds);
+    hipLaunchKernelGGL(k_iqrm_lag<uint16_t>, g, b, d.lds, st, argsof<LagArgs>(d));
+    break;
+  case KB_LAG_U32:
+    if (d.lds > 65536)
+      (void)hipFuncSetAttribute((const void *)k_iqrm_lag<uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)d.lds);
+    hipLaunchKernelGGL(k_iqrm_lag<uint32_t>, g, b, d.lds, st, argsof<LagArgs>(d));
+    break;
+  case KB_LAG_F32:
+    if (d.lds > 65536)
+      (void)hipFuncSetAttribute((const void *)k_iqrm_lag<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)d.lds);
+    hipLaunchKernelGGL(k_iqrm_lag<float>, g, b, d.lds, st, argsof<LagArgs>(d));
+    break;
+  case KB_MERGE:
+    if (d.lds > 65536)
+      (void)hipFuncSetAttribute((const void *)k_iqrm_merge, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)d.lds);
+    hipLaunchKernelGGL(k_iqrm_merge, g, b, d.lds, st, argsof<MergeArgs>(d));
+    break;
   case KB_SEGMASK: hipLaunchKernelGGL(k_segmask, g, b, d.lds, st, argsof<SegmaskArgs>(d)); break;
   case KB_CLEAN_U16: hipLaunchKernelGGL(k_clean<uint16_t>, g, b, d.lds, st, argsof<CleanArgs>(d)); break;
   case KB_CLEAN_U32: hipLaunchKernelGGL(k_clean<uint32_t>, g, b, d.lds, st, argsof<CleanArgs>(d)); break;
@@ -1416,6 +1480,9 @@ hipError_t launch_batched(int kind, const void *tab, const uint32_t *start, int 
   switch (kind) {
 #define AF_BL(K, KERN, TH, A)                                                        \
   case K:                                                                            \
+    if (lds > 65536)                                                                 \
+      (void)hipFuncSetAttribute((const void *)KERN,                                  \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
     hipLaunchKernelGGL(KERN, g, dim3(TH), lds, st, (const A *)tab, start, n);        \
     break;
     AF_BL(KB_BIN_U8, kb_bin_u8, 256, BinArgs)

@@ -173,35 +173,47 @@ void k_seg_stats_u8(StatArgs a) {
   stats_u8_body(a, blockIdx.x, gridDim.x);
 }
 
-// Sum the row-block partials of each segment: 64 channels x 16 item lanes
-// per block, combined in LDS.
+// Sum the row-block partials of each segment: 16 channels x 64 item lanes
+// per block (each wave reads four 128-byte runs per step), combined in LDS.
 __device__ __forceinline__ void reduce_u8_body(const ReduceArgs &ra, uint32_t bid, uint32_t) {
   const uint2 *__restrict__ part = (const uint2 *)ra.part;
   const int *__restrict__ seg_items = ra.seg_items;
   const int Csub = ra.Csub;
   Part *__restrict__ segsum = ra.segsum;
-  __shared__ uint64_t la[16][64], lb[16][64];
-  const int nchunk = (Csub + 63) / 64;
+  __shared__ uint64_t la[64][16], lb[64][16];
+  const int nchunk = (Csub + 15) / 16;
   const int s = (int)bid / nchunk;
-  const int c = ((int)bid % nchunk) * 64 + (threadIdx.x & 63);
-  const int il = threadIdx.x >> 6;
+  const int cl = threadIdx.x & 15;
+  const int c = ((int)bid % nchunk) * 16 + cl;
+  const int il = threadIdx.x >> 4;
   uint64_t a = 0, b = 0;
-  if (c < Csub)
-    for (int i = seg_items[s] + il; i < seg_items[s + 1]; i += 16) {
+  if (c < Csub) {
+    const int i1 = seg_items[s + 1];
+    int i = seg_items[s] + il;
+    // two loads in flight per step
+    for (; i + 64 < i1; i += 128) {
+      const uint2 p = part[(size_t)i * Csub + c], q = part[(size_t)(i + 64) * Csub + c];
+      a += (uint64_t)p.x + q.x;
+      b += (uint64_t)p.y + q.y;
+    }
+    for (; i < i1; i += 64) {
       const uint2 p = part[(size_t)i * Csub + c];
       a += p.x;
       b += p.y;
     }
-  la[il][threadIdx.x & 63] = a;
-  lb[il][threadIdx.x & 63] = b;
-  __syncthreads();
-  if (il == 0 && c < Csub) {
-    for (int k = 1; k < 16; ++k) {
-      a += la[k][threadIdx.x];
-      b += lb[k][threadIdx.x];
-    }
-    segsum[(size_t)s * Csub + c] = Part{a, b, 0};
   }
+  la[il][cl] = a;
+  lb[il][cl] = b;
+  __syncthreads();
+  // tree over the 64 item lanes
+  for (int h = 32; h > 0; h >>= 1) {
+    if (il < h) {
+      la[il][cl] += la[il + h][cl];
+      lb[il][cl] += lb[il + h][cl];
+    }
+    __syncthreads();
+  }
+  if (il == 0 && c < Csub) segsum[(size_t)s * Csub + c] = Part{la[0][cl], lb[0][cl], 0};
 }
 
 __global__ __launch_bounds__(1024) void k_seg_reduce_u8(ReduceArgs a) {
@@ -660,7 +672,7 @@ hipError_t launch_seg_stats_u8(const uint8_t *x, int nchan, int c0, int c1, cons
 
 hipError_t launch_seg_reduce_u8(const uint2 *part, const int *seg_items, int nseg, int Csub,
                                 Part *segsum, hipStream_t st) {
-  const int nchunk = (Csub + 63) / 64;
+  const int nchunk = (Csub + 15) / 16;   // 16 channels per block
   ReduceArgs a{part, seg_items, segsum, nseg, Csub};
   return emit_args(KB_RED_U8, 2, (size_t)nseg * nchunk, 1024, 0, a, st);
 }

@@ -389,6 +389,9 @@ static af_status build(af_plan *p) {
     p->iqrm_lds = (size_t)(2 * p->Csub + P2) * sizeof(float);
     if (p->iqrm_lds > (size_t)maxlds)
       return fail(AF_ERR_INVALID, "IQRM channel range too large for one workgroup");
+    // vote merge: casting counts + one bit per lag per channel
+    if ((size_t)p->Csub * (1 + (p->nlags + 31) / 32) * sizeof(int) > (size_t)maxlds)
+      return fail(AF_ERR_INVALID, "IQRM channel range x lags too large for one workgroup");
   }
   std::sort(bounds.begin(), bounds.end());
   bounds.erase(std::unique(bounds.begin(), bounds.end()), bounds.end());

@@ -86,6 +86,25 @@ def test_subband_u8_full_rfi(gpu_lib):
     assert np.array_equal(got, ref)
 
 
+def test_iqrm_many_lags(gpu_lib):
+    # radius 256 channels at geofactor 1.05: 164 lags, so the vote merge keeps
+    # six 32-bit words of flags per channel; the IQRM range [5, 290) also ends
+    # inside a 16-channel group of the statistics pass
+    say("case: direct u8 300ch, IQRM with 164 lags")
+    rng = np.random.default_rng(4)
+    b = small_band(300, 1200.0, 1500.0, 1e-4)
+    x = noise(rng, 4000, 300)
+    synth.inject_narrowband_rfi(x, rng, frac=0.08, amp=40, n_stuck=4, pmax=300)
+    spec = make_spec(x, b, 0.0, 5.0, 20, method="direct", use_iqrm=True, iqrm_c0=5,
+                     iqrm_c1=290, iqrm_mode=1, iqrm_radius_frac=0.9, iqrm_geofactor=1.05,
+                     iqrm_nsigma=2.0)
+    assert len(O.iqrm_lags(O.iqrm_radius(0.9, 285), 1.05)) > 128
+    got, ref, masks, rmask = run_both(x, spec)
+    assert masks.sum() > 0
+    assert np.array_equal(masks, rmask)
+    assert np.array_equal(got, ref)
+
+
 @pytest.mark.parametrize("ds", [2, 3])
 def test_subband_u8_downsampled(gpu_lib, ds):
     say(f"case: subband u8 ds={ds}")
