@@ -39,6 +39,7 @@ constexpr int kDdTile = 64 * kDdLanesOut;     // outputs per wave tile
 // own, so lane 63 only loads and its outputs are not kept.
 constexpr int kDdTileQ = 63 * kDdLanesOut;
 constexpr int kDdRowsPerBlock = 4;            // DM rows (waves) per workgroup
+constexpr int kDdRowsQ = 8;                   // DM rows (waves) per workgroup, aligned-load u8 kernel
 constexpr int kStatRows = 1024;               // rows per statistics item
 constexpr int kCleanRows = 64;                // rows per cleaning item
 constexpr size_t kInterBudget = size_t(1) << 30;   // subband intermediate per super-tile
@@ -118,10 +119,8 @@ struct DdArgs {
   int lds_rows;             // u8 LDS-staged kernel: DM rows per workgroup (0 = off)
   int lds_win;              // bytes of a channel's delay window staged in LDS
   const int4 *worklist;     // optional (grp, tile, rowblk, -) per workgroup, in launch order
-  const int2 *chan_order;   // u8: [nrows][nch] (channel, delay), each group's channels
-                            // sorted by (delay mod 16) / 4; null = natural order
-  const int *qstart;        // [nrows][ngrp][4] first position of each quarter class
   int nwork;
+  int u8q;                  // u8: the aligned-load kernel (kDdRowsQ rows in step, kDdTileQ tiles)
 };
 hipError_t launch_dedisperse(int dtype, const DdArgs &a, hipStream_t st);
 
@@ -274,10 +273,6 @@ struct af_plan {
   int dd_lds_rows = 0, dd_lds_win = 0;   // LDS-staged u8 dedispersion
   bool dd_q = false;                     // aligned-load u8 dedispersion (kDdTileQ tiles)
   std::vector<int4> worklist;            // stage-1 workgroups in locality order
-  std::vector<int2> chan_order;          // (channel, delay) sorted by delay quarter class
-  std::vector<int> qstart;
-  int2 *d_chan_order = nullptr;
-  int *d_qstart = nullptr;
   int4 *d_worklist = nullptr;
   double adds = 0;
   std::vector<int32_t> h_delays, h_res, h_maxres_j, h_maxres_jsb, h_sboff;

@@ -541,111 +541,101 @@ __global__ __launch_bounds__(256) void k_clean(CleanArgs a) {
 // the odd bytes; after at most 256 channels the even-byte sums are
 // S - (H<<8).  That is two adds per four samples (three with the byte
 // extraction), exact because no 16-bit lane can exceed 256*255.
-// Bytes [s, s+16) of the 32 bytes lo|hi (s wave-uniform, so the switch is a
-// scalar branch): four v_alignbyte_b32.
 __device__ __forceinline__ uint32_t ab(uint32_t h, uint32_t l, uint32_t r) {
   return __builtin_amdgcn_alignbyte(h, l, r);
 }
 
-__device__ __forceinline__ uint4 realign16(const uint4 lo, const uint4 hi, int s) {
-  const uint32_t r = (uint32_t)(s & 3);
-  switch (s >> 2) {
-  case 0: return make_uint4(ab(lo.y, lo.x, r), ab(lo.z, lo.y, r), ab(lo.w, lo.z, r), ab(hi.x, lo.w, r));
-  case 1: return make_uint4(ab(lo.z, lo.y, r), ab(lo.w, lo.z, r), ab(hi.x, lo.w, r), ab(hi.y, hi.x, r));
-  case 2: return make_uint4(ab(lo.w, lo.z, r), ab(hi.x, lo.w, r), ab(hi.y, hi.x, r), ab(hi.z, hi.y, r));
-  default: return make_uint4(ab(hi.x, lo.w, r), ab(hi.y, hi.x, r), ab(hi.z, hi.y, r), ab(hi.w, hi.z, r));
-  }
-}
-
-// Bytes [4q + r, 4q + r + 16) of lo|hi for a compile-time quarter q.
-template <int Q>
-__device__ __forceinline__ uint4 realign_q(const uint4 lo, const uint4 hi, uint32_t r) {
-  const uint32_t D[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-  return make_uint4(ab(D[Q + 1], D[Q], r), ab(D[Q + 2], D[Q + 1], r), ab(D[Q + 3], D[Q + 2], r),
-                    ab(D[Q + 4], D[Q + 3], r));
-}
-
-// Channels [i0, i1) of a row's sorted (channel, delay) list, all with delay
-// quarter class Q: one 16-byte aligned load per lane (an aligned lane load
-// costs the L1 a fraction of a misaligned one; each byte passes the L1 once),
-// the following 16 bytes from the neighbouring lane, and a fixed
+// Channels [c0, c1) of one DM row, 16 outputs per lane, into packed 16-bit
+// lane sums S/H.  Per channel every lane issues one 16-byte-aligned load (an
+// aligned lane load costs the L1 a fraction of a misaligned one); the 16
+// bytes after it are the right neighbour's chunk (DPP wave shift by one
+// lane), and the row's delay mod 16 (wave-uniform: a scalar branch) picks the
 // realignment.  Lane 63 has no right neighbour: it only loads, for lane 62,
-// and its sums are discarded (a wave keeps 63 x 16 outputs).
-template <int Q, bool O32>
-__device__ __forceinline__ void dd_class(const uint8_t *__restrict__ tb, size_t Lp, uint32_t loff,
-                                         const int2 *__restrict__ cd, int i0, int i1,
-                                         uint32_t (&S)[4], uint32_t (&H)[4]) {
-  // Row start of a (channel, delay) entry, wave-uniform: the channel's row
-  // plus the delay rounded down to 16 bytes.  O32: the buffer is below 4 GiB,
-  // so the offset is formed in 32 bits.
-  // O32: a raw buffer load, base tb, lane offset in a VGPR and the row
-  // offset in an SGPR (no 64-bit address arithmetic per channel)
+// and its sums are discarded (a wave keeps 63 x 16 outputs).  Eight loads are
+// issued before any is used.  SYNC: the workgroup's rows, which walk the same
+// channels, meet at a barrier every eight channels, so that their
+// overlapping windows are fetched into the L1 once.  O32: the channel-major
+// buffer is below 4 GiB, so raw buffer loads take the row offset in an SGPR.
+template <bool O32, bool SYNC>
+__device__ __forceinline__ void dd_q_channels(const uint8_t *__restrict__ tb, size_t Lp,
+                                              uint32_t loff, const int32_t *__restrict__ drow,
+                                              int c0, int c1, uint32_t (&S)[4],
+                                              uint32_t (&H)[4]) {
   const __amdgpu_buffer_rsrc_t rsrc =
       __builtin_amdgcn_make_buffer_rsrc((void *)tb, (short)0, -1, 0x00020000);
-  auto load = [&](const int2 e, uint4 &lo) {
+  auto load = [&](int c, int d, uint4 &lo) {
     if (O32) {
-      const uint32_t off = (uint32_t)e.x * (uint32_t)Lp + ((uint32_t)e.y & ~15u);
+      const uint32_t off = (uint32_t)c * (uint32_t)Lp + ((uint32_t)d & ~15u);
       const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)loff, (int)off, 0);
       lo = make_uint4(v[0], v[1], v[2], v[3]);
     } else {
-      const uint8_t *rb = tb + (size_t)e.x * Lp + (e.y & ~15);
+      const uint8_t *rb = tb + (size_t)c * Lp + (d & ~15);
       __builtin_memcpy(&lo, __builtin_assume_aligned(rb + loff, 16), 16);
     }
   };
-  // the next 16 bytes after a lane's chunk are the right neighbour's chunk
-  // (DPP wave shift left by one lane), then a fixed realignment
-  auto add = [&](const uint4 lo, const uint32_t r) {
-    // (bound_ctrl: lane 63, whose sums are discarded, reads zero)
-    uint4 hi;
-    hi.x = (uint32_t)__builtin_amdgcn_mov_dpp((int)lo.x, 0x130, 0xF, 0xF, true);
-    hi.y = (uint32_t)__builtin_amdgcn_mov_dpp((int)lo.y, 0x130, 0xF, 0xF, true);
-    hi.z = (uint32_t)__builtin_amdgcn_mov_dpp((int)lo.z, 0x130, 0xF, 0xF, true);
-    hi.w = (uint32_t)__builtin_amdgcn_mov_dpp((int)lo.w, 0x130, 0xF, 0xF, true);
-    const uint4 w = realign_q<Q>(lo, hi, r);
+  // (bound_ctrl: lane 63 reads zero)
+  auto dpp = [](uint32_t v) { return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x130, 0xF, 0xF, true); };
+  auto add = [&](const uint4 lo, const uint32_t s) {
+    const uint32_t r = s & 3u;
+    uint4 w;
+    switch (s >> 2) {
+    case 0: {
+      const uint32_t h0 = dpp(lo.x);
+      w = make_uint4(ab(lo.y, lo.x, r), ab(lo.z, lo.y, r), ab(lo.w, lo.z, r), ab(h0, lo.w, r));
+    } break;
+    case 1: {
+      const uint32_t h0 = dpp(lo.x), h1 = dpp(lo.y);
+      w = make_uint4(ab(lo.z, lo.y, r), ab(lo.w, lo.z, r), ab(h0, lo.w, r), ab(h1, h0, r));
+    } break;
+    case 2: {
+      const uint32_t h0 = dpp(lo.x), h1 = dpp(lo.y), h2 = dpp(lo.z);
+      w = make_uint4(ab(lo.w, lo.z, r), ab(h0, lo.w, r), ab(h1, h0, r), ab(h2, h1, r));
+    } break;
+    default: {
+      const uint32_t h0 = dpp(lo.x), h1 = dpp(lo.y), h2 = dpp(lo.z), h3 = dpp(lo.w);
+      w = make_uint4(ab(h0, lo.w, r), ab(h1, h0, r), ab(h2, h1, r), ab(h3, h2, r));
+    } break;
+    }
     S[0] += w.x; S[1] += w.y; S[2] += w.z; S[3] += w.w;
     H[0] += odd_bytes(w.x);
     H[1] += odd_bytes(w.y);
     H[2] += odd_bytes(w.z);
     H[3] += odd_bytes(w.w);
   };
-  // eight channels per step: the eight loads are issued before any is used
-  // (the DPP moves are convergent, so the unrolling is written out), then a
-  // remainder loop
-  // the (channel, delay) list is read-only for the kernel's lifetime:
-  // constant address space, so the wave-uniform reads are scalar loads
-  typedef __attribute__((address_space(4))) const uint64_t cu64;
-  cu64 *cdc = (cu64 *)cd;
-  auto entry = [&](int k) {
-    const uint64_t v = cdc[k];
-    return make_int2((int)(uint32_t)v, (int)(uint32_t)(v >> 32));
-  };
-  int i = i0;
-  for (; i + 8 <= i1; i += 8) {
+  // the delay table is read-only for the kernel's lifetime: constant address
+  // space, so the wave-uniform reads are scalar loads
+  typedef __attribute__((address_space(4))) const int32_t cint32;
+  cint32 *dc = (cint32 *)drow;
+  int c = c0;
+  for (; c + 8 <= c1; c += 8) {
     uint4 lo[8];
-    uint32_t r[8];
+    uint32_t sh[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      const int2 e = entry(i + u);
-      load(e, lo[u]);
-      r[u] = (uint32_t)e.y & 3u;
+      const int d = dc[c + u];
+      load(c + u, d, lo[u]);
+      sh[u] = (uint32_t)d & 15u;
     }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) add(lo[u], r[u]);
+    for (int u = 0; u < 8; ++u) add(lo[u], sh[u]);
+    if (SYNC) __builtin_amdgcn_s_barrier();
   }
-  for (; i < i1; ++i) {
-    const int2 e = entry(i);
+  for (; c < c1; ++c) {
+    const int d = dc[c];
     uint4 lo;
-    load(e, lo);
-    add(lo, (uint32_t)e.y & 3u);
+    load(c, d, lo);
+    add(lo, (uint32_t)d & 15u);
   }
 }
 
-// u8 dedispersion over quarter-class-sorted channel lists (same tiling and
-// packed-lane accumulation as k_dd_u8; channel blocks of <= 256 channels,
-// each summed in the four classes and flushed).
+// u8 dedispersion, aligned loads: a workgroup of kDdRowsQ waves owns
+// kDdRowsQ consecutive DM rows of one channel group and one tile of
+// kDdTileQ outputs; the rows walk the group's channels in step (see
+// dd_q_channels).  Channel blocks of <= 256 channels are flushed from the
+// packed lanes into 32-bit totals.
 template <bool O32>
 __device__ __forceinline__ void dd_u8_q_body(const DdArgs &a, uint32_t lid, uint32_t) {
-  const int nrowblk = (a.nrows + kDdRowsPerBlock - 1) / kDdRowsPerBlock;
+  const int nrowblk = (a.nrows + kDdRowsQ - 1) / kDdRowsQ;
   int rowblk, tile, grp;
   if (a.worklist) {
     const int4 wi = a.worklist[lid];
@@ -661,38 +651,47 @@ __device__ __forceinline__ void dd_u8_q_body(const DdArgs &a, uint32_t lid, uint
   }
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  const int row = rowblk * kDdRowsPerBlock + wave;
-  if (row >= a.nrows) return;
+  // every wave runs to the end (the rows meet at barriers): rows past the
+  // last compute row nrows - 1 again and store nothing
+  const int row_w = rowblk * kDdRowsQ + wave;
+  const bool row_ok = row_w < a.nrows;
+  const int row = row_ok ? row_w : a.nrows - 1;
   const int ca = grp * a.grp_w;
   const int cb = min(ca + a.grp_w, a.nch);
   const int blk_w = a.ngrp == 1 ? 256 : a.grp_w;
-  const int nblk_all = (a.nch + blk_w - 1) / blk_w;
   const uint8_t *__restrict__ tb = (const uint8_t *)a.xT + (size_t)tile * kDdTileQ;
   const uint32_t loff = (uint32_t)lane * kDdLanesOut;
-  const int2 *__restrict__ cd = a.chan_order + (size_t)row * a.nch;
+  const int32_t *__restrict__ drow = a.delays + (size_t)row * a.nch;
   uint32_t tot[16];
-#pragma unroll
-  for (int i = 0; i < 16; ++i) tot[i] = 0;
-  for (int b0 = ca; b0 < cb; b0 += blk_w) {
-    const int blk = b0 / blk_w;
-    const int bend = min(b0 + blk_w, cb);
-    const int *qs = a.qstart + ((size_t)row * nblk_all + blk) * 4;
-    const int q0 = qs[0], q1 = qs[1], q2 = qs[2], q3 = qs[3];
+  if (cb - ca <= blk_w) {
+    // one block (a subband): the totals are not live during the channel loop
     uint32_t S[4] = {0, 0, 0, 0}, H[4] = {0, 0, 0, 0};
-    dd_class<0, O32>(tb, a.Lp, loff, cd, q0, q1, S, H);
-    dd_class<1, O32>(tb, a.Lp, loff, cd, q1, q2, S, H);
-    dd_class<2, O32>(tb, a.Lp, loff, cd, q2, q3, S, H);
-    dd_class<3, O32>(tb, a.Lp, loff, cd, q3, bend, S, H);
+    dd_q_channels<O32, true>(tb, a.Lp, loff, drow, ca, cb, S, H);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const uint32_t L = S[k] - (H[k] << 8);
-      tot[4 * k + 0] += L & 0xFFFFu;
-      tot[4 * k + 1] += H[k] & 0xFFFFu;
-      tot[4 * k + 2] += L >> 16;
-      tot[4 * k + 3] += H[k] >> 16;
+      tot[4 * k + 0] = L & 0xFFFFu;
+      tot[4 * k + 1] = H[k] & 0xFFFFu;
+      tot[4 * k + 2] = L >> 16;
+      tot[4 * k + 3] = H[k] >> 16;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) tot[i] = 0;
+    for (int b0 = ca; b0 < cb; b0 += blk_w) {
+      uint32_t S[4] = {0, 0, 0, 0}, H[4] = {0, 0, 0, 0};
+      dd_q_channels<O32, true>(tb, a.Lp, loff, drow, b0, min(b0 + blk_w, cb), S, H);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t L = S[k] - (H[k] << 8);
+        tot[4 * k + 0] += L & 0xFFFFu;
+        tot[4 * k + 1] += H[k] & 0xFFFFu;
+        tot[4 * k + 2] += L >> 16;
+        tot[4 * k + 3] += H[k] >> 16;
+      }
     }
   }
-  if (lane == 63) return;
+  if (lane == 63 || !row_ok) return;
   const size_t t = (size_t)tile * kDdTileQ + (size_t)lane * kDdLanesOut;
   const size_t orow = ((size_t)row * a.ngrp + grp) * a.ostride;
   if (a.out16) {
@@ -719,7 +718,7 @@ __device__ __forceinline__ void dd_u8_q_body(const DdArgs &a, uint32_t lid, uint
   }
 }
 
-__global__ __launch_bounds__(256) void k_dd_u8_q(DdArgs a) {
+__global__ __launch_bounds__(64 * kDdRowsQ) void k_dd_u8_q(DdArgs a) {
   const uint32_t lid = xcd_remap(blockIdx.x, gridDim.x);
   if ((uint64_t)a.nch * a.Lp <= 0xFFFFFFFFull) dd_u8_q_body<true>(a, lid, gridDim.x);
   else dd_u8_q_body<false>(a, lid, gridDim.x);
@@ -1182,7 +1181,7 @@ AF_BATCHED(kb_clean_f32, CleanArgs, 256, (clean_body<float>(a, lid, nb)))
     CALL;                                                                               \
   }
 
-AF_BATCHED_REF(kb_dd_u8q, DdArgs, 256,
+AF_BATCHED_REF(kb_dd_u8q, DdArgs, 64 * kDdRowsQ,
                 ((uint64_t)a.nch * a.Lp <= 0xFFFFFFFFull ? dd_u8_q_body<true>(a, lid, nb)
                                                          : dd_u8_q_body<false>(a, lid, nb)))
 AF_BATCHED(kb_dd_u8, DdArgs, 256, (dd_u8_body(a, lid, nb)))
@@ -1302,7 +1301,9 @@ hipError_t launch_dedisperse(int dtype, const DdArgs &a, hipStream_t st) {
       const size_t lds = 2 * (size_t)1024 * 16;   // two buffers of one 16-byte chunk per thread
       return emit(KB_DD_U8LDS, 7, nblk16, 1024, lds, a, st);
     }
-    if (a.chan_order && (a.out16 || a.ngrp == 1)) return emit(KB_DD_U8Q, 7, nblk, 256, 0, a, st);
+    if (a.u8q && (a.out16 || a.ngrp == 1))
+      return emit(KB_DD_U8Q, 7, (size_t)((a.nrows + kDdRowsQ - 1) / kDdRowsQ) * a.ntiles * a.ngrp,
+                  64 * kDdRowsQ, 0, a, st);
     if (a.out16 || a.ngrp == 1) return emit(KB_DD_U8, 7, nblk, 256, 0, a, st);
     return emit(KB_DD_G_U8, 7, nblk, 256, 0, a, st);
   case AF_U16: return emit(KB_DD_G_U16, 7, nblk, 256, 0, a, st);
@@ -1504,7 +1505,7 @@ hipError_t launch_batched(int kind, const void *tab, const uint32_t *start, int 
     AF_BL(KB_CLEAN_U16, kb_clean_u16, 256, CleanArgs)
     AF_BL(KB_CLEAN_U32, kb_clean_u32, 256, CleanArgs)
     AF_BL(KB_CLEAN_F32, kb_clean_f32, 256, CleanArgs)
-    AF_BL(KB_DD_U8Q, kb_dd_u8q, 256, DdArgs)
+    AF_BL(KB_DD_U8Q, kb_dd_u8q, 64 * kDdRowsQ, DdArgs)
     AF_BL(KB_DD_U8, kb_dd_u8, 256, DdArgs)
     AF_BL(KB_DD_G_U8, kb_dd_g_u8, 256, DdArgs)
     AF_BL(KB_DD_G_U16, kb_dd_g_u16, 256, DdArgs)

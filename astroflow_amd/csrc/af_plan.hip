@@ -457,7 +457,7 @@ static af_status build(af_plan *p) {
   // once instead of once per tile.
   if (d.dtype == AF_U8) {
     const int rows = d.method == AF_METHOD_DIRECT ? p->ndm : p->ndm_nom;
-    const int JB = p->dd_lds_rows ? p->dd_lds_rows : kDdRowsPerBlock;
+    const int JB = p->dd_lds_rows ? p->dd_lds_rows : p->dd_q ? kDdRowsQ : kDdRowsPerBlock;
     const int nrb = (rows + JB - 1) / JB;
     const int ngrp = d.method == AF_METHOD_DIRECT ? 1 : p->nsb;
     const int gw = d.method == AF_METHOD_DIRECT ? p->nch : d.sb_nchan;
@@ -478,34 +478,8 @@ static af_status build(af_plan *p) {
     for (auto &kv : keyed) p->worklist.push_back(kv.second);
   }
 
-  // ---- channel order for the aligned-load u8 kernel: per DM row and channel
-  // block (<= 256 channels), channels sorted by the quarter (delay mod 16)/4
-  // of their delay, with the start of each class.
-  if (p->dd_q) {
-    const int rows = d.method == AF_METHOD_DIRECT ? p->ndm : p->ndm_nom;
-    const int bw = d.method == AF_METHOD_DIRECT ? 256 : d.sb_nchan;
-    const int nblk = (p->nch + bw - 1) / bw;
-    p->chan_order.resize((size_t)rows * p->nch);
-    p->qstart.resize((size_t)rows * nblk * 4);
-    for (int r = 0; r < rows; ++r) {
-      const int32_t *dr = &p->h_delays[(size_t)r * p->nch];
-      int2 *out = &p->chan_order[(size_t)r * p->nch];
-      for (int b = 0; b < nblk; ++b) {
-        const int c0 = b * bw, c1 = std::min(c0 + bw, p->nch);
-        int pos = c0;
-        for (int q = 0; q < 4; ++q) {
-          p->qstart[((size_t)r * nblk + b) * 4 + q] = pos;
-          for (int c = c0; c < c1; ++c)
-            if (((dr[c] & 15) >> 2) == q) out[pos++] = int2{c, dr[c]};
-        }
-      }
-    }
-  }
-
   // ---- device memory
   AF_TRY(upload(p, &p->d_worklist, p->worklist));
-  AF_TRY(upload(p, &p->d_chan_order, p->chan_order));
-  AF_TRY(upload(p, &p->d_qstart, p->qstart));
   AF_TRY(upload(p, &p->d_delays, p->h_delays));
   AF_TRY(upload(p, &p->d_res, p->h_res));
   AF_TRY(upload(p, &p->d_maxres_j, p->h_maxres_j));
@@ -650,8 +624,7 @@ af_status run_dedisp(af_plan *p, const void *xT, size_t Lp, void *d_out, hipStre
   da.lds_rows = p->dd_lds_rows;
   da.lds_win = p->dd_lds_win;
   da.worklist = p->d_worklist;
-  da.chan_order = p->d_chan_order;
-  da.qstart = p->d_qstart;
+  da.u8q = p->dd_q ? 1 : 0;
   da.nwork = (int)p->worklist.size();
   if (d.method == AF_METHOD_DIRECT) {
     da.nrows = p->ndm;
