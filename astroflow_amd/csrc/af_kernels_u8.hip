@@ -240,13 +240,25 @@ __device__ __forceinline__ uint32_t zdm4(uint32_t w, float m) {
   return o;
 }
 
+// zdm4 when (uint8)trunc(max(x - m, 0)) = max(x - k, 0) for every byte x,
+// k = ceil(m): true when m is an integer or frac(m) >= 2^-16.  (x - m is
+// then exact or rounded by at most 2^-17 < frac(m): it never reaches the next
+// integer.)  Even and odd bytes as 16-bit lanes, saturating subtract.
+__device__ __forceinline__ uint32_t zsub4(uint32_t w, uint32_t kk) {
+  typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+  const us2 k2 = __builtin_bit_cast(us2, kk);
+  const us2 e = __builtin_elementwise_sub_sat(__builtin_bit_cast(us2, w & 0x00FF00FFu), k2);
+  const us2 o = __builtin_elementwise_sub_sat(__builtin_bit_cast(us2, (w >> 8) & 0x00FF00FFu), k2);
+  return __builtin_bit_cast(uint32_t, e) | (__builtin_bit_cast(uint32_t, o) << 8);
+}
+
 __device__ __forceinline__ uint32_t nonzero_bytes(uint32_t w) {
   const uint32_t t = ((w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | w;   // bit 7 of each byte: byte != 0
   return __builtin_popcount(t & 0x80808080u);
 }
 
 __device__ __forceinline__ void clean_u8_body(const CleanArgs &a, uint32_t bid, uint32_t) {
-  __shared__ uint32_t s_sum[32], s_cnt[32];
+  __shared__ uint32_t s_sum[32], s_cnt[32], s_kk[32];
   __shared__ float s_m[32];
   if (a.run_if && *a.run_if == 0) return;
   const RowItem it = a.items[bid];
@@ -325,17 +337,31 @@ __device__ __forceinline__ void clean_u8_body(const CleanArgs &a, uint32_t bid, 
     __syncthreads();
     if (threadIdx.x < 32) {
       const uint32_t n = s_cnt[threadIdx.x];
-      s_m[threadIdx.x] = n ? (float)s_sum[threadIdx.x] / (float)n : 0.0f;
+      const float m = n ? (float)s_sum[threadIdx.x] / (float)n : 0.0f;
+      s_m[threadIdx.x] = m;
+      // the byte subtract of zsub4 applies (kk != 0xFFFFFFFF)
+      const float fl = floorf(m), fr = m - fl;   // exact: m < 256
+      const uint32_t k = (uint32_t)fl + (fr > 0.0f ? 1u : 0u);
+      s_kk[threadIdx.x] = (fr == 0.0f || fr >= 0x1p-16f) ? (k | (k << 16)) : 0xFFFFFFFFu;
     }
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-      const float m = s_m[8 * q + i];
+      const uint32_t kk = s_kk[8 * q + i];
       uint32_t o[4];
+      if (kk != 0xFFFFFFFFu) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const uint32_t v = u4get(w[i], k);
-        o[k] = (zdm4(v, m) & zr[k]) | (v & ~zr[k]);
+        for (int k = 0; k < 4; ++k) {
+          const uint32_t v = u4get(w[i], k);
+          o[k] = (zsub4(v, kk) & zr[k]) | (v & ~zr[k]);
+        }
+      } else {
+        const float m = s_m[8 * q + i];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const uint32_t v = u4get(w[i], k);
+          o[k] = (zdm4(v, m) & zr[k]) | (v & ~zr[k]);
+        }
       }
       w[i] = make_uint4(o[0] & st[0], o[1] & st[1], o[2] & st[2], o[3] & st[3]);
     }
