@@ -44,6 +44,13 @@ constexpr int kStatRows = 1024;               // rows per statistics item
 constexpr int kCleanRows = 64;                // rows per cleaning item
 constexpr size_t kInterBudget = size_t(1) << 30;   // subband intermediate per super-tile
 
+// Read-only int table (delays, residuals, LDS row offsets) through the
+// constant address space: a wave-uniform index becomes a scalar load.
+__device__ __forceinline__ int32_t ldc(const int32_t *p, size_t i) {
+  typedef __attribute__((address_space(4))) const int32_t ci32;
+  return ((ci32 *)p)[i];
+}
+
 size_t dtype_size(int dtype);
 int acc_size(int dtype);  // bytes of one output element
 

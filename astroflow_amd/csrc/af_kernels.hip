@@ -73,22 +73,23 @@ __device__ __forceinline__ void fill_rows16(uint4 *__restrict__ lds, int ldsw,
                                             const E *__restrict__ src, size_t pitch, int epc,
                                             int nrows, int ext) {
   const int total = nrows * ext;
+  if (total <= 0) return;
   const int step = (int)blockDim.x;
+  // a thread past the end repeats the last chunk (same value, same slot):
+  // loads and stores stay unconditional, so the four loads stay in flight
+  // (a load under a per-lane branch is waited for at the join)
   for (int base = threadIdx.x; base < total; base += 4 * step) {
+    int slot[4];
     uint4 v[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int idx = base + u * step;
-      if (idx < total) {
-        const int r = idx / ext, k = idx % ext;
-        v[u] = *(const uint4 *)(src + (size_t)r * pitch + (size_t)epc * k);
-      }
+      const int idx = min(base + u * step, total - 1);
+      const int r = idx / ext, k = idx - r * ext;
+      slot[u] = r * ldsw + k;
+      v[u] = *(const uint4 *)(src + (size_t)r * pitch + (size_t)epc * k);
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int idx = base + u * step;
-      if (idx < total) lds[(idx / ext) * ldsw + idx % ext] = v[u];
-    }
+    for (int u = 0; u < 4; ++u) lds[slot[u]] = v[u];
   }
 }
 
@@ -1003,7 +1004,7 @@ __device__ __forceinline__ void stage2_u16_body(const Stage2Args &a, uint32_t li
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0;
     for (int sb = 0; sb < a.nsb; ++sb) {
-      const uint16_t *p = s2 + sb * a.lds_len + lane * per + rrow[sb];
+      const uint16_t *p = s2 + sb * a.lds_len + lane * per + ldc(rrow, sb);
 #pragma unroll
       for (int i = 0; i < 16; ++i)
         if (i < per) acc[i] += p[i];
@@ -1075,7 +1076,7 @@ __device__ __forceinline__ void stage2_f32c_body(const Stage2Args &a, uint32_t l
       if (mi < a.ndm0 && m < a.ndm) {
         const int32_t *__restrict__ rrow = a.res + (size_t)m * a.nsb + sb0;
         for (int sb = 0; sb < nsc; ++sb) {
-          const float *p = s2 + sb * a.lds_len + lane + rrow[sb];   // aligned 4-byte reads
+          const float *p = s2 + sb * a.lds_len + lane + ldc(rrow, sb);   // aligned 4-byte reads
 #pragma unroll
           for (int i = 0; i < P; ++i) acc[d][i] += (double)p[64 * i];
         }

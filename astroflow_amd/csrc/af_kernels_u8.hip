@@ -30,6 +30,34 @@ __device__ __forceinline__ uint32_t add_hi16(uint32_t acc, uint32_t v) {
   return __builtin_amdgcn_udot2(__builtin_bit_cast(u16x2_t, v), (u16x2_t){0, 1}, acc, false);
 }
 
+// Sums over the subbands of one fine DM from LDS: P outputs per lane, lane l
+// taking outputs l, l+64, ...; row sb of the tile starts at s2 + off[sb] and
+// the DM's residual shifts it by rrow[sb].  Eight subbands per step: their
+// offsets are scalar loads issued together, then 8 x P aligned 2-byte LDS
+// reads.
+template <int P>
+__device__ __forceinline__ void s2_sum_dm(const uint16_t *s2, const int32_t *off,
+                                          const int32_t *rrow, int nsb, int lane,
+                                          uint32_t (&acc)[P]) {
+  int sb = 0;
+  for (; sb + 8 <= nsb; sb += 8) {
+    int o[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) o[u] = ldc(off, sb + u) + ldc(rrow, sb + u);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const uint16_t *p = s2 + o[u] + lane;
+#pragma unroll
+      for (int i = 0; i < P; ++i) acc[i] += p[64 * i];
+    }
+  }
+  for (; sb < nsb; ++sb) {
+    const uint16_t *p = s2 + ldc(off, sb) + ldc(rrow, sb) + lane;
+#pragma unroll
+    for (int i = 0; i < P; ++i) acc[i] += p[64 * i];
+  }
+}
+
 __device__ __forceinline__ uint32_t u4get(const uint4 &v, int k) {
   return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
 }
@@ -52,22 +80,23 @@ __device__ __forceinline__ void fill_rows16(uint4 *__restrict__ lds, int ldsw,
                                             const E *__restrict__ src, size_t pitch, int epc,
                                             int nrows, int ext) {
   const int total = nrows * ext;
+  if (total <= 0) return;
   const int step = (int)blockDim.x;
+  // a thread past the end repeats the last chunk (same value, same slot):
+  // loads and stores stay unconditional, so the four loads stay in flight
+  // (a load under a per-lane branch is waited for at the join)
   for (int base = threadIdx.x; base < total; base += 4 * step) {
+    int slot[4];
     uint4 v[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int idx = base + u * step;
-      if (idx < total) {
-        const int r = idx / ext, k = idx % ext;
-        v[u] = *(const uint4 *)(src + (size_t)r * pitch + (size_t)epc * k);
-      }
+      const int idx = min(base + u * step, total - 1);
+      const int r = idx / ext, k = idx - r * ext;
+      slot[u] = r * ldsw + k;
+      v[u] = *(const uint4 *)(src + (size_t)r * pitch + (size_t)epc * k);
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int idx = base + u * step;
-      if (idx < total) lds[(idx / ext) * ldsw + idx % ext] = v[u];
-    }
+    for (int u = 0; u < 4; ++u) lds[slot[u]] = v[u];
   }
 }
 
@@ -437,34 +466,47 @@ __device__ __forceinline__ void stage2_u16v_body(const Stage2Args &a, uint32_t l
   // four loads are issued before their LDS stores.
   {
     int sb = wave, k0 = 0;
-    int o8 = sb < a.nsb ? off[sb] / 8 : 0;
-    int n8 = sb < a.nsb ? off[sb + 1] / 8 - o8 : 0;
-    while (sb < a.nsb) {
-      uint4 v[4];
-      int dst[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        dst[u] = -1;
-        if (sb < a.nsb) {
-          const int k = k0 + lane;
-          if (k < n8) {
-            v[u] = *(const uint4 *)(src + (size_t)sb * a.L1 + 8 * (size_t)k);
-            dst[u] = o8 + k;
-          }
-          k0 += 64;
-          if (k0 >= n8) {
-            sb += nw;
-            k0 = 0;
-            if (sb < a.nsb) {
-              o8 = off[sb] / 8;
-              n8 = off[sb + 1] / 8 - o8;
-            }
+    int o8 = sb < a.nsb ? ldc(off, sb) / 8 : 0;
+    int n8 = sb < a.nsb ? ldc(off, sb + 1) / 8 - o8 : 0;
+    const int spare = a.lds_len / 8;
+    // one (row, pass) slot of the walk (wave-uniform): the lane's chunk
+    // address (a lane past its row re-reads the row's last chunk) and LDS
+    // slot (-1: none)
+    auto slot = [&](const uint16_t *&ptr, int &dst) {
+      ptr = src;
+      dst = -1;
+      if (sb < a.nsb) {
+        const int k = k0 + lane;
+        ptr = src + (size_t)sb * a.L1 + 8 * (size_t)min(k, n8 - 1);
+        dst = k < n8 ? o8 + k : -1;
+        k0 += 64;
+        if (k0 >= n8) {
+          sb += nw;
+          k0 = 0;
+          if (sb < a.nsb) {
+            o8 = ldc(off, sb) / 8;
+            n8 = ldc(off, sb + 1) / 8 - o8;
           }
         }
       }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (dst[u] >= 0) s2v[dst[u]] = v[u];
+    };
+    // four slots, then four unconditional loads, then the LDS stores (a load
+    // under a per-lane branch is waited for at the join)
+    while (sb < a.nsb) {
+      const uint16_t *p0, *p1, *p2, *p3;
+      int d0, d1, d2, d3;
+      slot(p0, d0);
+      slot(p1, d1);
+      slot(p2, d2);
+      slot(p3, d3);
+      const uint4 v0 = *(const uint4 *)p0, v1 = *(const uint4 *)p1;
+      const uint4 v2 = *(const uint4 *)p2, v3 = *(const uint4 *)p3;
+      // unconditional stores (a spare chunk past the tile takes the unused
+      // ones), so the loads are not moved into per-lane branches
+      s2v[d0 >= 0 ? d0 : spare] = v0;
+      s2v[d1 >= 0 ? d1 : spare] = v1;
+      s2v[d2 >= 0 ? d2 : spare] = v2;
+      s2v[d3 >= 0 ? d3 : spare] = v3;
     }
   }
   __syncthreads();
@@ -478,11 +520,7 @@ __device__ __forceinline__ void stage2_u16v_body(const Stage2Args &a, uint32_t l
     uint32_t acc[P];
 #pragma unroll
     for (int i = 0; i < P; ++i) acc[i] = 0;
-    for (int sb = 0; sb < a.nsb; ++sb) {
-      const uint16_t *p = s2 + off[sb] + lane + rrow[sb];
-#pragma unroll
-      for (int i = 0; i < P; ++i) acc[i] += p[64 * i];
-    }
+    s2_sum_dm<P>(s2, off, rrow, a.nsb, lane, acc);
     uint32_t *o = (uint32_t *)a.out + (size_t)m * a.ostride + t0 + (size_t)lane;
 #pragma unroll
     for (int i = 0; i < P; ++i)
@@ -531,7 +569,7 @@ __device__ __forceinline__ void stage2_u16c_body(const Stage2Args &a, uint32_t l
       if (mi < a.ndm0 && m < a.ndm) {
         const int32_t *__restrict__ rrow = a.res + (size_t)m * a.nsb + sb0;
         for (int sb = 0; sb < nsc; ++sb) {
-          const uint16_t *p = s2 + sb * a.lds_len + lane + rrow[sb];
+          const uint16_t *p = s2 + sb * a.lds_len + lane + ldc(rrow, sb);
 #pragma unroll
           for (int i = 0; i < P; ++i) acc[d][i] += p[64 * i];
         }
@@ -718,7 +756,7 @@ hipError_t launch_stage2_u16v(const Stage2Args &a, hipStream_t st) {
   }
   // all subbands: packed rows, a.lds_len u16 elements per workgroup
   if (!a.sboff || (P != 4 && P != 8 && P != 16)) return hipErrorInvalidValue;
-  const size_t lds = (size_t)a.lds_len * sizeof(uint16_t);
+  const size_t lds = (size_t)a.lds_len * sizeof(uint16_t) + 16;   // + the spare chunk
   return emit_args(P == 16 ? KB_S2_V16 : P == 8 ? KB_S2_V8 : KB_S2_V4, 8, nblk, 512, lds, a, st);
 }
 

@@ -256,7 +256,7 @@ static af_status build(af_plan *p) {
             e += (size_t)((t + p->h_maxres_jsb[(size_t)j * p->nsb + sb] + 7) / 8 * 8);
           best = std::max(best, e);
         }
-        return best * 2;
+        return best * 2 + 16;   // + one spare 16-byte chunk
       };
       bool packed = false;
       // a requested stage-2 tile (autotuner), when it fits
@@ -279,7 +279,7 @@ static af_status build(af_plan *p) {
           for (int sb = 0; sb < p->nsb; ++sb)
             o[sb + 1] = o[sb] + (tile + p->h_maxres_jsb[(size_t)j * p->nsb + sb] + 7) / 8 * 8;
         }
-        p->s2_pk = (int)(lds_pk(tile) / 2);
+        p->s2_pk = (int)((lds_pk(tile) - 16) / 2);
       }
       // many subbands: stage them in chunks (<= 64 KB per workgroup)
       if (!tile && d.ndm0 <= 32) {
