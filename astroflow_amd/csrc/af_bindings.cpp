@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "af_image.h"
+#include "af_snr.h"
 #include "astroflow_amd.h"
 
 namespace py = pybind11;
@@ -638,6 +639,59 @@ void def_result(py::module_ &m, const char *name) {
       .def_readonly("filname", &D::filname);
 }
 
+// _calculate_frb_snr (src/bindpy/frb_snr.cpp:144-398): the same argument
+// order, defaults and return tuple (snr, width, peak, (mean, std, info)).
+py::tuple calculate_frb_snr(py::array spec, py::object noise_range, double threshold_sigma,
+                            py::object toa_sample_idx, py::object fitting_window_samples,
+                            py::object tsamp, py::object target_time_us) {
+  auto a = py::array_t<float, py::array::c_style | py::array::forcecast>::ensure(spec);
+  if (!a) throw std::runtime_error("spec must be a 2D numpy array");
+  if (a.ndim() != 2) throw std::runtime_error("spec must be 2D (time x frequency)");
+  afsnr::SnrInput in;
+  in.spec = a.data();
+  in.ntime = (int)a.shape(0);
+  in.nfreq = (int)a.shape(1);
+  in.sigma = threshold_sigma;
+  if (!noise_range.is_none() && py::len(noise_range) > 0) {
+    in.has_noise_ranges = true;
+    for (py::handle item : noise_range) {
+      py::sequence pr = py::reinterpret_borrow<py::sequence>(item);
+      if (py::len(pr) != 2) continue;
+      in.noise_ranges.emplace_back(pr[0].cast<int>(), pr[1].cast<int>());
+    }
+  }
+  if (!toa_sample_idx.is_none()) {
+    in.has_toa = true;
+    in.toa = toa_sample_idx.cast<int>();
+  }
+  if (!fitting_window_samples.is_none()) {
+    in.has_window = true;
+    in.window = fitting_window_samples.cast<int>();
+  }
+  if (!tsamp.is_none()) {
+    in.has_tsamp = true;
+    in.tsamp = tsamp.cast<double>();
+  }
+  if (!target_time_us.is_none()) {
+    in.has_target = true;
+    in.target_us = target_time_us.cast<double>();
+  }
+  const afsnr::SnrResult r = afsnr::boxcar_snr(in);
+  py::dict q;
+  q["fit_converged"] = r.converged;
+  if (r.empty) return py::make_tuple(-1.0, 0, 0, py::make_tuple(0.0, 1.0, q));
+  if (r.fallback) {
+    q["method"] = "fallback";
+    return py::make_tuple(r.snr, r.width, r.peak, py::make_tuple(r.noise_mean, r.noise_std, q));
+  }
+  q["method"] = "boxcar";
+  q["width_samples"] = r.width;
+  q["search_start"] = r.search_start;
+  q["search_end"] = r.search_end;
+  q["downsample"] = r.downsample;
+  return py::make_tuple(r.snr, r.width, r.peak, py::make_tuple(r.noise_mean, r.noise_std, q));
+}
+
 }  // namespace
 
 PYBIND11_MODULE(_astroflow_core, m) {
@@ -733,6 +787,11 @@ PYBIND11_MODULE(_astroflow_core, m) {
         py::arg("time_downsample"), py::arg("t_sample"), py::arg("target") = 1,
         py::arg("target_id") = 0, py::arg("maskfile") = "mask.txt",
         py::arg("rficfg") = RFIConfig());
+
+  m.def("_calculate_frb_snr", &calculate_frb_snr, py::arg("spec"),
+        py::arg("noise_range") = py::none(), py::arg("threshold_sigma") = 5.0,
+        py::arg("toa_sample_idx") = py::none(), py::arg("fitting_window_samples") = py::none(),
+        py::arg("tsamp") = py::none(), py::arg("target_time_us") = 256.0);
 
   m.def("_image_slice", [](py::array_t<float, py::array::c_style> plane, int size) {
     // one DM-time image from a float plane (exposed for tests of the imaging)
