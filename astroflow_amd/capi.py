@@ -45,6 +45,7 @@ class PlanDesc(C.Structure):
         ("use_zero_dm", C.c_int), ("zdm_c0", C.c_int), ("zdm_c1", C.c_int),
         ("bad_chans", C.POINTER(C.c_int)), ("n_bad", C.c_int),
         ("tile_t", C.c_int), ("dm_per_block", C.c_int), ("inter_budget", C.c_size_t),
+        ("clean_budget", C.c_size_t),
     ]
 
 
@@ -52,7 +53,8 @@ class PlanInfo(C.Structure):
     _fields_ = [("nbin", C.c_size_t), ("nout", C.c_size_t), ("ndm", C.c_int),
                 ("out_elem_bytes", C.c_int), ("nwin", C.c_int), ("nsb", C.c_int),
                 ("ndm_nom", C.c_int), ("max_delay", C.c_int), ("scratch_bytes", C.c_size_t),
-                ("adds", C.c_double), ("inter_bytes", C.c_size_t)]
+                ("adds", C.c_double), ("inter_bytes", C.c_size_t),
+                ("clean_passes", C.c_int)]
 
 
 class DeviceProps(C.Structure):
@@ -216,6 +218,7 @@ class DedispSpec:
     tile_t: int = 0
     dm_per_block: int = 0
     inter_budget: int = 0
+    clean_budget: int = 0
 
     def to_desc(self):
         """Returns (PlanDesc, keepalive) -- keepalive owns the arrays it points to."""
@@ -245,6 +248,7 @@ class DedispSpec:
         d.n_bad = len(bad)
         d.tile_t, d.dm_per_block = self.tile_t, self.dm_per_block
         d.inter_budget = self.inter_budget
+        d.clean_budget = self.clean_budget
         return d, (freq, bad)
 
 
@@ -353,10 +357,20 @@ class DeviceBuffer:
         assert a.nbytes <= self.nbytes
         check(lib().af_memcpy_h2d(C.c_void_p(self.ptr), _ptr(a), a.nbytes, None), "h2d")
 
-    def download(self, a: np.ndarray):
-        assert a.flags.c_contiguous and a.nbytes <= self.nbytes
-        check(lib().af_memcpy_d2h(_ptr(a), C.c_void_p(self.ptr), a.nbytes, None), "d2h")
+    def download(self, a: np.ndarray, offset: int = 0):
+        """Fills `a` from the bytes at `offset` of this buffer."""
+        assert a.flags.c_contiguous and offset >= 0 and offset + a.nbytes <= self.nbytes
+        check(lib().af_memcpy_d2h(_ptr(a), C.c_void_p(self.ptr + offset), a.nbytes, None), "d2h")
         return a
+
+    def upload_at(self, a: np.ndarray, offset: int):
+        """Copies `a` to the bytes at `offset` of this buffer."""
+        a = np.ascontiguousarray(a)
+        assert offset >= 0 and offset + a.nbytes <= self.nbytes
+        check(lib().af_memcpy_h2d(C.c_void_p(self.ptr + offset), _ptr(a), a.nbytes, None), "h2d")
+
+    def zero(self):
+        check(lib().af_memset(C.c_void_p(self.ptr), 0, self.nbytes, None), "memset")
 
     def free(self):
         if self.ptr:

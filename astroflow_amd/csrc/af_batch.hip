@@ -115,17 +115,22 @@ af_status af_batch_create(af_plan *const *plans, int nplans, const void *const *
     std::memcpy(host.data() + off, src, n);
     return off;
   };
-  std::vector<bool> tiled(nplans, false);
+  // A plan that cleans per super-tile reuses its cleaned buffer between
+  // those passes too: all of its launches stay single, in its own order,
+  // after everything else.
+  std::vector<bool> tiled(nplans, false), ordered(nplans, false);
   for (int i = 0; i < nplans; ++i) {
     int n7 = 0;
     for (auto &d : recs[i]) n7 += d.stage == 7;
     tiled[i] = n7 > 1;
+    ordered[i] = plans[i]->clean_tiled;
   }
   auto emit_group = [&](int stage, int kind) -> af_status {
     std::vector<const LaunchDesc *> group;
     for (int i = 0; i < nplans; ++i)
       for (auto &d : recs[i])
-        if (d.stage == stage && d.kind == kind && !(tiled[i] && (stage == 7 || stage == 8)))
+        if (!ordered[i] && d.stage == stage && d.kind == kind &&
+            !(tiled[i] && (stage == 7 || stage == 8)))
           group.push_back(&d);
     if (group.empty()) return AF_OK;
     if (!batchable(kind) || group.size() < 2) {
@@ -172,7 +177,7 @@ af_status af_batch_create(af_plan *const *plans, int nplans, const void *const *
     }
     if (stage == 8)
       for (int i = 0; i < nplans; ++i)
-        if (tiled[i])
+        if (tiled[i] && !ordered[i])
           for (auto &d : recs[i])
             if (d.stage == 7 || d.stage == 8) {
               Merged m;
@@ -181,6 +186,14 @@ af_status af_batch_create(af_plan *const *plans, int nplans, const void *const *
               b->launches.push_back(m);
             }
   }
+  for (int i = 0; i < nplans; ++i)
+    if (ordered[i])
+      for (auto &d : recs[i]) {
+        Merged m;
+        m.kind = d.kind;
+        m.single = d;
+        b->launches.push_back(m);
+      }
   if (!host.empty()) {
     if (hipMalloc(&b->d_tables, host.size()) != hipSuccess) {
       release(b);

@@ -216,7 +216,7 @@ enum LaunchKind : int {
   KB_DD_G_F32D,
   KB_S2_V4, KB_S2_V8, KB_S2_V16, KB_S2_C8, KB_S2_C16, KB_S2_U16, KB_S2_G_U16, KB_S2_G_U32, KB_S2_G_U64, KB_S2_G_F32,
   KB_S2_F32C8, KB_S2_F32C16,
-  KB_COPY,
+  KB_COPY, KB_ZERO,
   KB_COUNT
 };
 
@@ -291,13 +291,19 @@ struct af_plan {
   std::vector<af::Seg> segs;
   std::vector<af::RowItem> stat_items, clean_items;
   std::vector<int> seg_items, lags;
+  // cleaning per super-tile: d_xT holds one super-tile's rows [k*T2,
+  // k*T2 + Lp) (channel-major, row pitch Lp); tile_items[tile_off[k] ..
+  // tile_off[k+1]) are the clean items clipped to those rows
+  bool clean_tiled = false;
+  std::vector<af::RowItem> tile_items;
+  std::vector<int> tile_off;
 
   // device buffers
   int32_t *d_delays = nullptr, *d_res = nullptr, *d_maxres_j = nullptr, *d_sboff = nullptr;
   int *d_lags = nullptr, *d_seg_items = nullptr;
   uint32_t *d_wseg = nullptr, *d_wt0 = nullptr, *d_wt1 = nullptr;
   af::Seg *d_segs = nullptr;
-  af::RowItem *d_stat_items = nullptr, *d_clean_items = nullptr;
+  af::RowItem *d_stat_items = nullptr, *d_clean_items = nullptr, *d_tile_items = nullptr;
   uint8_t *d_static = nullptr;
   void *d_xb = nullptr, *d_xT = nullptr, *d_inter = nullptr;
   af::Part *d_part = nullptr, *d_segsum = nullptr;
@@ -345,10 +351,13 @@ hipError_t launch_copy_rows(uint8_t *dst, const uint8_t *src, size_t pitch, size
                             int nrows, hipStream_t st);
 // Zeroes width bytes of each of nrows rows spaced pitch bytes apart.
 hipError_t launch_zero_rows(uint8_t *base, size_t pitch, size_t width, int nrows, hipStream_t st);
+// launch_zero_rows through the launch recorder (a batch replays it in order)
+hipError_t launch_zero(uint8_t *base, size_t pitch, size_t width, int nrows, hipStream_t st);
 
 // Pieces of the plan's execute (af_plan.hip), for callers that keep their
 // own cleaned data: IQRM vote from p->d_segsum into p->d_masks / d_segmask,
 // and both dedispersion stages from a channel-major cleaned buffer.
 af_status run_iqrm_vote(af_plan *p, hipStream_t st);
-af_status run_dedisp(af_plan *p, const void *xT, size_t Lp, void *d_out, hipStream_t st, int set);
+af_status run_dedisp(af_plan *p, const void *xT, size_t Lp, void *d_out, hipStream_t st, int set,
+                     int k_only = -1);
 }  // namespace af

@@ -1354,6 +1354,18 @@ hipError_t launch_gather(int dtype, const void *xT, size_t Lp, const int32_t *de
   return hipGetLastError();
 }
 
+struct ZeroArgs {
+  uint8_t *base;
+  size_t pitch, width;
+  int nrows;
+};
+
+hipError_t launch_zero(uint8_t *base, size_t pitch, size_t width, int nrows, hipStream_t st) {
+  if (width == 0 || nrows <= 0) return hipSuccess;
+  ZeroArgs a{base, pitch, width, nrows};
+  return emit(KB_ZERO, 6, 1, 1, 0, a, st);
+}
+
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st) {
   if (bytes == 0) return hipSuccess;
   CopyArgs a{dst, src, bytes};
@@ -1466,6 +1478,10 @@ hipError_t launch_desc(const LaunchDesc &d, hipStream_t st) {
     const CopyArgs &c = argsof<CopyArgs>(d);
     return hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToDevice, st);
   }
+  case KB_ZERO: {
+    const ZeroArgs &z = argsof<ZeroArgs>(d);
+    return launch_zero_rows(z.base, z.pitch, z.width, z.nrows, st);
+  }
   default:
     return launch_desc_u8(d, st);
   }
@@ -1473,7 +1489,8 @@ hipError_t launch_desc(const LaunchDesc &d, hipStream_t st) {
 }
 
 bool batchable(int kind) {
-  return kind != KB_DD_U8LDS && kind != KB_COPY && kind >= 0 && kind < KB_COUNT;
+  return kind != KB_DD_U8LDS && kind != KB_COPY && kind != KB_ZERO && kind >= 0 &&
+         kind < KB_COUNT;
 }
 
 hipError_t launch_batched(int kind, const void *tab, const uint32_t *start, int n,

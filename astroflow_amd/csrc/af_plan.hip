@@ -366,6 +366,35 @@ static af_status build(af_plan *p) {
                     (size_t)p->dd_lds_win;
   p->Lp = std::max(p->nbin, need_len);
   p->Lp = (p->Lp + 255) / 256 * 256;
+  // Cleaning per super-tile: the cleaned buffer then holds one super-tile's
+  // reach (L1 + the largest delay + the load overhang) instead of the whole
+  // input, at the cost of cleaning the overlap rows once per super-tile.
+  if (d.method == AF_METHOD_SUBBAND && p->nsuper > 1) {
+    const size_t es = dtype_size(d.dtype);
+    const size_t lt = ((size_t)p->L1 + (size_t)p->maxdelay + 64 + (size_t)p->dd_lds_win + 255) /
+                      256 * 256;
+    const size_t whole = (size_t)p->nch * p->Lp * es, one = (size_t)p->nch * lt * es;
+    bool tiled = false;
+    if (d.clean_budget) {
+      if (whole > d.clean_budget) {
+        if (one > d.clean_budget)
+          return fail(AF_ERR_NOMEM, "clean_budget is below one super-tile of cleaned rows (" +
+                                        std::to_string(one) + " bytes)");
+        tiled = true;
+      }
+    } else {
+      size_t fr = 0, tot = 0;
+      if (hipMemGetInfo(&fr, &tot) == hipSuccess) {
+        const size_t other = (ds > 1 ? p->nbin * (size_t)d.nchan * es : 0) +
+                             (size_t)p->ndm_nom * p->nsb * p->L1 * p->inter_elem + (size_t(1) << 30);
+        tiled = one < whole && whole + other > fr;
+      }
+    }
+    if (tiled) {
+      p->clean_tiled = true;
+      p->Lp = lt;
+    }
+  }
 
   // ---- IQRM windows, lags and row segmentation
   std::vector<uint32_t> bounds = {0u, (uint32_t)p->nbin};
@@ -449,6 +478,18 @@ static af_status build(af_plan *p) {
       p->clean_items.push_back(RowItem{(uint32_t)s, r, std::min<uint32_t>(r + clean_rows, p->segs[s].t1), 0});
   }
 
+  if (p->clean_tiled) {
+    p->tile_off.assign(1, 0);
+    for (int k = 0; k < p->nsuper; ++k) {
+      const size_t a = (size_t)k * p->T2, b = std::min(p->nbin, a + p->Lp);
+      for (const RowItem &it : p->clean_items) {
+        const size_t lo = std::max<size_t>(it.r0, a), hi = std::min<size_t>(it.r1, b);
+        if (lo < hi) p->tile_items.push_back(RowItem{it.seg, (uint32_t)lo, (uint32_t)hi, 0});
+      }
+      p->tile_off.push_back((int)p->tile_items.size());
+    }
+  }
+
   // ---- stage-1 work order.  Workgroup (group g, tile t, row block b) reads
   // channel c from t*T + d(first row of b, c) onwards.  Sorting each group's
   // workgroups by that start (at the group's middle channel) makes the
@@ -492,6 +533,7 @@ static af_status build(af_plan *p) {
   AF_TRY(upload(p, &p->d_segs, p->segs));
   AF_TRY(upload(p, &p->d_stat_items, p->stat_items));
   AF_TRY(upload(p, &p->d_clean_items, p->clean_items));
+  if (p->clean_tiled) AF_TRY(upload(p, &p->d_tile_items, p->tile_items));
   if (!p->bad.empty()) {
     std::vector<uint8_t> st(d.nchan, 0);
     bool any = false;
@@ -549,7 +591,10 @@ static af_status collect(af_plan *p) {
 
 // Binning, IQRM, window masks, zero-DM and static mask into the channel-
 // major buffer p->d_xT (src/core/dedispered.cu:1050-1118).
-static af_status run_clean(af_plan *p, const void *d_in, hipStream_t st, int set) {
+// With do_clean false, only binning, the statistics and the vote run (the
+// clean pass is then issued per super-tile by execute()).
+static af_status run_clean(af_plan *p, const void *d_in, hipStream_t st, int set,
+                           bool do_clean = true) {
   const af_plan_desc &d = p->d;
   AF_TRY(record(p, set, 0, st));
   const void *src = d_in;
@@ -567,6 +612,7 @@ static af_status run_clean(af_plan *p, const void *d_in, hipStream_t st, int set
   AF_TRY(record(p, set, 2, st));
   if (iq) AF_TRY(run_iqrm_vote(p, st));
   AF_TRY(record(p, set, 3, st));
+  if (!do_clean) return AF_OK;
   CleanArgs ca{};
   ca.x = src;
   ca.xT = p->d_xT;
@@ -593,8 +639,42 @@ af_status execute(af_plan *p, const void *d_in, void *d_out, uint8_t *d_masks, h
     if ((size_t)(p->ev_used + 1) * af_plan::kNumEv > p->ev.size()) AF_TRY(collect(p));
     set = p->ev_used++;
   }
-  AF_TRY(run_clean(p, d_in, st, set));
-  AF_TRY(run_dedisp(p, p->d_xT, p->Lp, d_out, st, set));
+  if (!p->clean_tiled) {
+    AF_TRY(run_clean(p, d_in, st, set));
+    AF_TRY(run_dedisp(p, p->d_xT, p->Lp, d_out, st, set));
+  } else {
+    // per super-tile k: clean rows [k*T2, k*T2 + Lp) into d_xT (row r at
+    // column r - k*T2), zero the columns past the input, dedisperse.  Stage
+    // times: "dedisp" then holds the clean passes and both stages.
+    AF_TRY(run_clean(p, d_in, st, set, false));
+    AF_TRY(record(p, set, 4, st));
+    const size_t es = dtype_size(d.dtype);
+    const bool iq = d.use_iqrm && p->nwin > 0;
+    for (int k = 0; k < p->nsuper; ++k) {
+      const size_t a = (size_t)k * p->T2;
+      CleanArgs ca{};
+      ca.x = d.ds > 1 ? p->d_xb : d_in;
+      ca.xT = (void *)((uintptr_t)p->d_xT - (uintptr_t)(a * es));
+      ca.Lp = p->Lp;
+      ca.nchan = d.nchan;
+      ca.items = p->d_tile_items + p->tile_off[k];
+      ca.nitems = p->tile_off[k + 1] - p->tile_off[k];
+      ca.segmask = iq ? p->d_segmask : nullptr;
+      ca.stat_mask = p->d_static;
+      ca.zdm = d.use_zero_dm && d.zdm_c1 > d.zdm_c0;
+      ca.zc0 = d.zdm_c0;
+      ca.zc1 = d.zdm_c1;
+      ca.dd_c0 = d.dd_c0;
+      ca.dd_c1 = d.dd_c1;
+      if (ca.nitems > 0) AF_HIP(launch_clean(d.dtype, ca, st));
+      if (a + p->Lp > p->nbin) {
+        // columns past the input read as zero (earlier super-tiles wrote there)
+        const size_t c0 = p->nbin > a ? p->nbin - a : 0;
+        AF_HIP(launch_zero((uint8_t *)p->d_xT + c0 * es, p->Lp * es, (p->Lp - c0) * es, p->nch, st));
+      }
+      AF_TRY(run_dedisp(p, p->d_xT, p->Lp, d_out, st, k + 1 == p->nsuper ? set : -1, k));
+    }
+  }
   if (d_masks && d.use_iqrm && p->nwin > 0)
     AF_HIP(launch_copy(d_masks, p->d_masks, (size_t)p->nwin * d.nchan, st));
   return AF_OK;
@@ -613,7 +693,9 @@ af_status run_iqrm_vote(af_plan *p, hipStream_t st) {
   return AF_OK;
 }
 
-af_status run_dedisp(af_plan *p, const void *xT, size_t Lp, void *d_out, hipStream_t st, int set) {
+// k_only >= 0: super-tile k_only alone, with xT holding its rows from column 0.
+af_status run_dedisp(af_plan *p, const void *xT, size_t Lp, void *d_out, hipStream_t st, int set,
+                     int k_only) {
   const af_plan_desc &d = p->d;
   DdArgs da{};
   da.xT = xT;
@@ -663,9 +745,10 @@ af_status run_dedisp(af_plan *p, const void *xT, size_t Lp, void *d_out, hipStre
     sa.sboff = p->d_sboff;
     // super-tile k: stage 1 reads the cleaned buffer from column k*T2,
     // stage 2 writes output columns [k*T2, min(nout, (k+1)*T2))
-    for (int k = 0; k < p->nsuper; ++k) {
+    const int k0 = k_only >= 0 ? k_only : 0, k1 = k_only >= 0 ? k_only + 1 : p->nsuper;
+    for (int k = k0; k < k1; ++k) {
       const size_t t0 = (size_t)k * p->T2;
-      da.xT = (const uint8_t *)xT + t0 * es;
+      da.xT = (const uint8_t *)xT + (k_only >= 0 ? 0 : t0 * es);
       AF_HIP(launch_dedisperse(d.dtype, da, st));
       if (k + 1 == p->nsuper) AF_TRY(record(p, set, 5, st));
       sa.out = (uint8_t *)d_out + t0 * os;
@@ -731,6 +814,7 @@ af_status af_plan_query(const af_plan *p, af_plan_info *info) {
   info->scratch_bytes = p->scratch_bytes;
   info->adds = p->adds;
   info->inter_bytes = p->nsb > 0 ? (size_t)p->ndm_nom * p->nsb * p->L1 * p->inter_elem : 0;
+  info->clean_passes = p->clean_tiled ? p->nsuper : 1;
   return AF_OK;
 }
 

@@ -473,7 +473,11 @@ af_status af_server_add_beam(af_server *s, const af_plan_desc *w, size_t chunk_r
   }
   AF_HIP(hipSetDevice(s->device));
   Beam b;
-  AF_TRY(af_plan_create(w, &b.plan));
+  // the beam keeps its own channel-major window: its plan never cleans per
+  // super-tile
+  af_plan_desc wd = *w;
+  wd.clean_budget = ~(size_t)0;
+  AF_TRY(af_plan_create(&wd, &b.plan));
   af_plan_info info;
   af_plan_query(b.plan, &info);
   b.row_bytes = (size_t)w->nchan * dtype_size(w->dtype);

@@ -151,6 +151,42 @@ def chunk_4096(seed: int = 2, threads: int = 8):
     return x, band, nout, dmax
 
 
+def direct_1024(seed: int = 1, threads: int = 8):
+    """The plain direct-method case: 1024 channels over 1000-1500 MHz (top
+    first, flipped to ascending), tsamp 49.152 us, 2^18 samples of u8
+    N(128,16) noise and one dispersed Gaussian pulse (DM 500, FWHM 1 ms,
+    +3 sigma per channel).  Returns (x [2^18][1024] u8, band, pulse t0)."""
+    band = band_1000_1500(1024)
+    nsamp = 1 << 18
+    x = noise_u8_threaded(seed, nsamp, 1024, threads=threads)
+    t0 = 60000
+    inject_pulse(x, band.freq, band.f_top, float(band.tsamp), 500.0, t0, 1e-3, 48.0)
+    return x, band, t0
+
+
+# The three-step DM plan of the large float32 plane: (dm_low, dm_step, ndm,
+# time downsample) -- DM 0-1000 step 0.5 at full rate, 1000-4000 step 1 at 2x
+# downsampling, 4000-10000 step 2 at 4x.
+C5_STEPS = ((0.0, 0.5, 2001, 1), (1000.0, 1.0, 3001, 2), (4000.0, 2.0, 3001, 4))
+
+
+def inject_impulsive_rfi(x: np.ndarray, rng: np.random.Generator, frac=0.02, smin=5.0,
+                         smax=20.0, lmin=1, lmax=50):
+    """Broadband bursts (every channel) of 1-50 samples at 5-20 sigma, placed
+    until about `frac` of the time samples are hit.  Returns the burst list."""
+    n = x.shape[0]
+    hit = 0
+    bursts = []
+    while hit < frac * n:
+        ln = int(rng.integers(lmin, lmax + 1))
+        t0 = int(rng.integers(0, max(1, n - ln)))
+        amp = float(rng.uniform(smin, smax))
+        x[t0:t0 + ln] += np.float32(amp)
+        bursts.append((t0, ln, amp))
+        hit += ln
+    return bursts
+
+
 def c3_chunks(seed: int = 4, n: int = 48):
     """48 independent beam-chunks of the batched workload: nchan in {1024,
     2048, 4096}, nsamp log-uniform in [2^14, 2^20], dtype uint8 or float32,

@@ -84,6 +84,13 @@ typedef struct af_plan_desc {
   int dm_per_block;        /* DM rows per stage-1 workgroup: 4 or 16 (0 = 4)    */
   size_t inter_budget;     /* bytes of subband intermediate per time super-tile
                               (0 = 1 GiB); longer outputs run in super-tiles  */
+  size_t clean_budget;     /* bytes of the cleaned channel-major buffer (subband
+                              method with several super-tiles): when the whole
+                              buffer exceeds it, each super-tile's rows (plus
+                              the DM-overlap reach) are cleaned just before its
+                              dedispersion into a buffer of one super-tile.
+                              0 = whole buffer when it fits the free device
+                              memory, else per super-tile                     */
 } af_plan_desc;
 
 typedef struct af_plan af_plan;
@@ -100,6 +107,8 @@ typedef struct af_plan_info {
   size_t scratch_bytes;    /* device memory held by the plan                    */
   double adds;             /* integer/float adds the method performs per call   */
   size_t inter_bytes;      /* subband intermediate written by stage 1 per call  */
+  int clean_passes;        /* 1: one clean pass over the whole buffer; >1: one per
+                              super-tile (see clean_budget)                    */
 } af_plan_info;
 
 /* Per-stage device time, averaged over the n executes run since profiling

@@ -298,7 +298,7 @@ def run_pipeline(x: np.ndarray, spec: PlanSpec, flavour=IQRM_GPU):
     if spec.ds > 1:
         x = bin_time(x, spec.ds)
     nbin = x.shape[0]
-    nout = nbin if spec.nout is None else spec.nout
+    nout = spec.nout or nbin      # 0 or None: the full binned length
     masks = np.zeros((0, spec.nchan), np.uint8)
     t0s = t1s = np.zeros(0, np.uint32)
     if spec.use_iqrm:
@@ -321,6 +321,38 @@ def run_pipeline(x: np.ndarray, spec: PlanSpec, flavour=IQRM_GPU):
                              spec.tsamp, spec.ds, spec.dm_low, spec.dm_step, spec.ndm, spec.ndm0)
         plane = dedisperse_subband(x, spec.dd_c0, spec.dd_c1, spec.sbw, d1, res, spec.ndm0, nout)
     return plane, masks, t0s, t1s, x
+
+
+def run_pipeline_rows(x: np.ndarray, spec: PlanSpec, rows, flavour=IQRM_GPU):
+    """run_pipeline for the DM rows `rows` only (large inputs).  Returns
+    (plane rows, masks, cleaned)."""
+    x = np.array(x, copy=True, order="C")
+    if spec.ds > 1:
+        x = bin_time(x, spec.ds)
+    nbin = x.shape[0]
+    nout = spec.nout or nbin      # 0 or None: the full binned length
+    masks = np.zeros((0, spec.nchan), np.uint8)
+    if spec.use_iqrm:
+        masks, t0s, t1s = iqrm(x, spec.iqrm_c0, spec.iqrm_c1, spec.iqrm_mode,
+                               spec.iqrm_radius_frac, spec.iqrm_nsigma, spec.iqrm_geofactor,
+                               spec.iqrm_W, spec.iqrm_H, spec.iqrm_include_tail, flavour)
+        apply_window_masks(x, masks, t0s, t1s)
+    if spec.use_zero_dm:
+        zero_dm(x, spec.zdm_c0, spec.zdm_c1)
+    if len(spec.bad_channels):
+        static_mask(x, spec.bad_channels)
+    if spec.method == "direct":
+        d = delay_table(spec.freq, spec.dd_c0, spec.dd_c1, spec.f_ref, spec.tsamp, spec.ds,
+                        spec.dms())
+        out = dedisperse_direct_rows(x, spec.dd_c0, spec.dd_c1, d, nout, rows)
+    else:
+        d1 = delay_table(spec.freq, spec.dd_c0, spec.dd_c1, spec.f_ref, spec.tsamp, spec.ds,
+                         spec.coarse_dms())
+        res = residual_table(spec.freq, spec.dd_c0, spec.dd_c1, spec.sbw, spec.f_ref,
+                             spec.tsamp, spec.ds, spec.dm_low, spec.dm_step, spec.ndm, spec.ndm0)
+        out = dedisperse_subband_rows(x, spec.dd_c0, spec.dd_c1, spec.sbw, d1, res, spec.ndm0,
+                                      nout, rows)
+    return out, masks, x
 
 
 # ---------------------------------------------------- reference (oracle/_ref) --

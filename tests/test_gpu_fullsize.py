@@ -59,3 +59,40 @@ def test_bench_workload_full_size(gpu_lib):
         r = plane[m].astype(np.float64)
         assert (r.max() - np.median(r)) > 8 * r.std(), m
     plan.close()
+
+
+def test_direct_1024ch_full_size(gpu_lib):
+    # 1024 ch x 2^18 u8 samples, DM 0-1000 step 1 (1001 trials), direct
+    # method, zero-DM over [0, 1023), full-length u32 plane
+    print("case: full-size 1024ch x 2^18 direct + zero-DM", flush=True)
+    from astroflow_amd import capi, synth
+    from tests.helpers import make_spec
+    x, band, t0 = synth.direct_1024(seed=1)
+    nsamp, nchan = x.shape
+    spec = make_spec(x, band, 0.0, 1.0, 1001, method="direct",
+                     use_zero_dm=True, zdm_c0=0, zdm_c1=nchan - 1)
+    plan = capi.Plan(spec)
+    d_in = capi.DeviceBuffer(x.nbytes)
+    d_in.upload(x)
+    d_out = capi.DeviceBuffer(1001 * nsamp * 4)
+    plan.execute(d_in.ptr, d_out.ptr)
+    capi.sync()
+    plane = d_out.download(np.empty((1001, nsamp), np.uint32))
+    d_in.free()
+    d_out.free()
+    plan.close()
+    print("gpu done; oracle rows", flush=True)
+    s = oracle_spec(spec)
+    cleaned = x.copy()
+    O.zero_dm(cleaned, s.zdm_c0, s.zdm_c1)
+    f = s.freq
+    d = O.delay_table(f, 0, nchan, f[-1], s.tsamp, 1, np.arange(1001, dtype=np.float64))
+    rows = [0, 1, 250, 499, 500, 501, 999, 1000]
+    ref = O.dedisperse_direct_rows(cleaned, 0, nchan, d, nsamp, rows)
+    for i, m in enumerate(rows):
+        assert np.array_equal(plane[m], ref[i]), f"row {m} differs"
+    assert np.array_equal(plane[0], cleaned.astype(np.uint32).sum(1))
+    # the pulse peaks at DM 500 at its arrival time at the top channel
+    r = plane[500].astype(np.float64)
+    assert abs(int(np.argmax(r)) - t0) <= 2
+    assert (r.max() - np.median(r)) > 8 * r.std()

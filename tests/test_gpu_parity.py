@@ -210,3 +210,53 @@ def test_subband_u8_many_subbands_large_halo(gpu_lib, sbw, tile):
                      use_zero_dm=True, zdm_c0=0, zdm_c1=2048)
     got, ref, *_ = run_both(x, spec, want_masks=False)
     assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("dtype,ds", [(np.uint8, 1), (np.uint8, 2), (np.uint16, 1),
+                                      (np.uint32, 1), (np.float32, 1), (np.float32, 4)])
+def test_clean_per_super_tile(gpu_lib, dtype, ds):
+    # a clean budget below the whole channel-major buffer: each super-tile's
+    # rows are cleaned just before its dedispersion (the overlap rows once per
+    # super-tile, columns past the input zeroed); the plane and the masks equal
+    # the oracle's and the whole-buffer plan's
+    say(f"case: clean per super-tile, {np.dtype(dtype).name} ds {ds}")
+    rng = np.random.default_rng(31)
+    b = small_band(384, 1100.0, 1500.0, 1e-4)
+    x = noise(rng, 24000, 384, dtype)
+    if dtype == np.uint8:
+        synth.inject_narrowband_rfi(x, rng, frac=0.05, amp=40.0, n_stuck=2, pmax=512)
+    kw = dict(method="subband", sbw=64, ndm0=16, ds=ds, inter_budget=10 * 6 * 2 * 4096,
+              use_iqrm=True, iqrm_c0=0, iqrm_c1=383, iqrm_mode=1, iqrm_W=7000 // ds,
+              iqrm_H=5000 // ds, use_zero_dm=True, zdm_c0=0, zdm_c1=383, bad_channels=[5, 77])
+    whole = make_spec(x, b, 0.0, 1.0, 150, **kw)
+    pw = capi.Plan(whole)
+    es = np.dtype(dtype).itemsize
+    assert pw.info.clean_passes == 1
+    budget = 384 * es * (2 * pw.info.nbin // 3)
+    pw.close()
+    tiled = make_spec(x, b, 0.0, 1.0, 150, clean_budget=budget, **kw)
+    pt = capi.Plan(tiled)
+    assert pt.info.clean_passes > 2, pt.info.clean_passes
+    got, masks = pt.run_host(x, want_masks=True)
+    pt.close()
+    pw = capi.Plan(whole)
+    got_w, masks_w = pw.run_host(x, want_masks=True)
+    pw.close()
+    assert np.array_equal(masks, masks_w)
+    assert np.array_equal(got, got_w)
+    ref, rmask, *_ = O.run_pipeline(x, oracle_spec(tiled), O.IQRM_GPU)
+    assert np.array_equal(masks, rmask)
+    if dtype == np.float32:
+        assert_f32_close(got, ref, x, tiled)
+    else:
+        assert np.array_equal(got, ref)
+
+
+def test_clean_budget_below_one_super_tile_is_rejected(gpu_lib):
+    rng = np.random.default_rng(32)
+    b = small_band(256, 1100.0, 1500.0, 1e-4)
+    x = noise(rng, 20000, 256)
+    spec = make_spec(x, b, 0.0, 1.0, 100, method="subband", sbw=64, ndm0=16,
+                     inter_budget=10 * 6 * 2 * 4096, clean_budget=4096)
+    with pytest.raises(MemoryError):
+        capi.Plan(spec)
