@@ -328,6 +328,7 @@ af_status execute(af_plan *p, const void *d_in, void *d_out, uint8_t *d_masks, h
 
 // 8-bit fast paths (af_kernels_u8.hip)
 constexpr int kStatRowsU8 = 256;   // rows per statistics item (4 row lanes x 64)
+constexpr int kStatRowsU8Max = 1024;   // rows per plain-call statistics item (<= 256 per row lane)
 constexpr int kCleanRowsU8 = 32;   // rows per clean item (1024 threads x 8 rows x 16 ch)
 constexpr int kCleanU8MaxChan = 4096;
 hipError_t launch_seg_stats_u8(const uint8_t *x, int nchan, int c0, int c1, const RowItem *items,

@@ -547,13 +547,14 @@ __device__ __forceinline__ uint32_t ab(uint32_t h, uint32_t l, uint32_t r) {
 }
 
 // Channels [c0, c1) of one DM row, 16 outputs per lane, into packed 16-bit
-// lane sums S/H.  Per channel every lane issues one 16-byte-aligned load (an
-// aligned lane load costs the L1 a fraction of a misaligned one); the 16
-// bytes after it are the right neighbour's chunk (DPP wave shift by one
-// lane), and the row's delay mod 16 (wave-uniform: a scalar branch) picks the
-// realignment.  Lane 63 has no right neighbour: it only loads, for lane 62,
-// and its sums are discarded (a wave keeps 63 x 16 outputs).  Eight loads are
-// issued before any is used.  SYNC: the workgroup's rows, which walk the same
+// lane sums S/H.  Per channel every lane issues one 16-byte load at the
+// delay rounded down to a dword (a dword-aligned lane load costs the L1 no
+// more than a 16-byte-aligned one, a byte-misaligned one about one access
+// per lane); the dword after it is the right neighbour's first dword (DPP
+// wave shift by one lane), and the row's delay mod 4 (wave-uniform, in an
+// SGPR) is the byte realignment: no branch on the delay.  Lane 63 has no
+// right neighbour: it only loads, for lane 62, and its sums are discarded (a
+// wave keeps 63 x 16 outputs).  Eight loads are issued before any is used.  SYNC: the workgroup's rows, which walk the same
 // channels, meet at a barrier every eight channels, so that their
 // overlapping windows are fetched into the L1 once.  O32: the channel-major
 // buffer is below 4 GiB, so raw buffer loads take the row offset in an SGPR.
@@ -566,37 +567,20 @@ __device__ __forceinline__ void dd_q_channels(const uint8_t *__restrict__ tb, si
       __builtin_amdgcn_make_buffer_rsrc((void *)tb, (short)0, -1, 0x00020000);
   auto load = [&](int c, int d, uint4 &lo) {
     if (O32) {
-      const uint32_t off = (uint32_t)c * (uint32_t)Lp + ((uint32_t)d & ~15u);
+      const uint32_t off = (uint32_t)c * (uint32_t)Lp + ((uint32_t)d & ~3u);
       const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)loff, (int)off, 0);
       lo = make_uint4(v[0], v[1], v[2], v[3]);
     } else {
-      const uint8_t *rb = tb + (size_t)c * Lp + (d & ~15);
-      __builtin_memcpy(&lo, __builtin_assume_aligned(rb + loff, 16), 16);
+      const uint8_t *rb = tb + (size_t)c * Lp + (d & ~3);
+      __builtin_memcpy(&lo, __builtin_assume_aligned(rb + loff, 4), 16);
     }
   };
   // (bound_ctrl: lane 63 reads zero)
   auto dpp = [](uint32_t v) { return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x130, 0xF, 0xF, true); };
-  auto add = [&](const uint4 lo, const uint32_t s) {
-    const uint32_t r = s & 3u;
-    uint4 w;
-    switch (s >> 2) {
-    case 0: {
-      const uint32_t h0 = dpp(lo.x);
-      w = make_uint4(ab(lo.y, lo.x, r), ab(lo.z, lo.y, r), ab(lo.w, lo.z, r), ab(h0, lo.w, r));
-    } break;
-    case 1: {
-      const uint32_t h0 = dpp(lo.x), h1 = dpp(lo.y);
-      w = make_uint4(ab(lo.z, lo.y, r), ab(lo.w, lo.z, r), ab(h0, lo.w, r), ab(h1, h0, r));
-    } break;
-    case 2: {
-      const uint32_t h0 = dpp(lo.x), h1 = dpp(lo.y), h2 = dpp(lo.z);
-      w = make_uint4(ab(lo.w, lo.z, r), ab(h0, lo.w, r), ab(h1, h0, r), ab(h2, h1, r));
-    } break;
-    default: {
-      const uint32_t h0 = dpp(lo.x), h1 = dpp(lo.y), h2 = dpp(lo.z), h3 = dpp(lo.w);
-      w = make_uint4(ab(h0, lo.w, r), ab(h1, h0, r), ab(h2, h1, r), ab(h3, h2, r));
-    } break;
-    }
+  auto add = [&](const uint4 lo, const uint32_t r) {
+    const uint32_t h0 = dpp(lo.x);
+    const uint4 w =
+        make_uint4(ab(lo.y, lo.x, r), ab(lo.z, lo.y, r), ab(lo.w, lo.z, r), ab(h0, lo.w, r));
     S[0] += w.x; S[1] += w.y; S[2] += w.z; S[3] += w.w;
     H[0] += odd_bytes(w.x);
     H[1] += odd_bytes(w.y);
@@ -615,7 +599,7 @@ __device__ __forceinline__ void dd_q_channels(const uint8_t *__restrict__ tb, si
     for (int u = 0; u < 8; ++u) {
       const int d = dc[c + u];
       load(c + u, d, lo[u]);
-      sh[u] = (uint32_t)d & 15u;
+      sh[u] = (uint32_t)d & 3u;
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u) add(lo[u], sh[u]);
@@ -625,7 +609,7 @@ __device__ __forceinline__ void dd_q_channels(const uint8_t *__restrict__ tb, si
     const int d = dc[c];
     uint4 lo;
     load(c, d, lo);
-    add(lo, (uint32_t)d & 15u);
+    add(lo, (uint32_t)d & 3u);
   }
 }
 

@@ -454,8 +454,10 @@ static af_status build(af_plan *p) {
     p->wseg[2 * w + 1] = (uint32_t)b + 1;
   }
   // u8 statistics: two 1024-thread workgroups fit a CU; rows per item are
-  // chosen so the items fill the machine in two equal rounds (a few hundred
-  // large items leave a short, mostly idle last round)
+  // chosen so the items fill the machine in one round (up to 1024 rows per
+  // item: each thread then sums at most 256 rows in its packed 16-bit lanes).
+  // Measured on the 4096-ch chunk: one round of 296-row items beats two of
+  // 148 and four of 74 (fewer per-item reductions).
   uint32_t stat_rows = d.dtype == AF_U8 ? kStatRowsU8 : kStatRows;
   if (d.dtype == AF_U8) {
     int ncu = 0;
@@ -463,9 +465,9 @@ static af_status build(af_plan *p) {
         ncu <= 0)
       ncu = 256;
     const size_t slots = 2 * (size_t)ncu;
-    size_t r = (p->nbin + 2 * slots - 1) / (2 * slots);
+    size_t r = (p->nbin + slots - 1) / slots;
     r = (r + 3) / 4 * 4;
-    stat_rows = (uint32_t)std::min<size_t>(kStatRowsU8, std::max<size_t>(16, r));
+    stat_rows = (uint32_t)std::min<size_t>(kStatRowsU8Max, std::max<size_t>(16, r));
   }
   const uint32_t clean_rows =
       (d.dtype == AF_U8 && d.nchan <= kCleanU8MaxChan) ? kCleanRowsU8 : kCleanRows;
