@@ -81,7 +81,9 @@ typedef struct af_plan_desc {
   int n_bad;
   /* kernel tuning; 0 = library default (the result does not depend on these) */
   int tile_t;              /* stage-2 output tile: 64..1024 (0 = chosen by LDS)  */
-  int dm_per_block;        /* DM rows per stage-1 workgroup: 4 or 16 (0 = 4)    */
+  int dm_per_block;        /* u8 stage-1 form: 0 = 8 DM rows per workgroup, each
+                              loading its windows through the L1; 16 = 16 rows
+                              sharing one LDS window per channel              */
   size_t inter_budget;     /* bytes of subband intermediate per time super-tile
                               (0 = 1 GiB); longer outputs run in super-tiles  */
   size_t clean_budget;     /* bytes of the cleaned channel-major buffer (subband
