@@ -89,9 +89,9 @@ hipError_t launch_iqrm_lags(int dtype, const Part *segsum, const Seg *segs,
                             const uint32_t *wt1, int nwin, int Csub, int mode,
                             const int *lags, int nlags, float nsigma,
                             uint8_t *flags, size_t lds_bytes, hipStream_t st);
-hipError_t launch_iqrm_merge(const uint8_t *flags, const int *lags, int nlags,
-                             int nwin, int Csub, int c0, int nchan,
-                             uint8_t *masks, hipStream_t st);
+hipError_t launch_iqrm_merge(const uint8_t *flags, const int *lags, int nlags, int nwin,
+                             int Csub, int c0, int nchan, uint8_t *masks, int reach,
+                             hipStream_t st);
 hipError_t launch_segmask(const uint8_t *masks, const Seg *segs, int nseg,
                           int nchan, uint8_t *segmask, hipStream_t st);
 
@@ -286,6 +286,7 @@ struct af_plan {
 
   // IQRM / segmentation
   int nwin = 0, nseg = 0, nlags = 0, Csub = 0;
+  int lag_reach = 0;        // largest |lag| below Csub (the vote's channel reach)
   size_t iqrm_lds = 0;
   std::vector<uint32_t> wt0, wt1, wseg;
   std::vector<af::Seg> segs;

@@ -212,9 +212,91 @@ __global__ __launch_bounds__(256) void k_seg_reduce(ReduceArgs a) {
 }
 
 // --------------------------------------------------- IQRM robust voting --
+// One bitonic pass at distance J < E between a thread's own elements.
+template <int E, int J>
+__device__ __forceinline__ void sort_pass_regs(float (&v)[E], int t, int kk) {
+  if constexpr (J < E) {
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      if ((r & J) == 0) {
+        const bool asc = ((E * t + r) & kk) == 0;
+        const float a = v[r], b = v[r | J];
+        const float mn = fminf(a, b), mx = fmaxf(a, b);
+        v[r] = asc ? mn : mx;
+        v[r | J] = asc ? mx : mn;
+      }
+    }
+  }
+}
+
+// Lagged differences of the statistic (src/core/iqrm.cu:94-117) into ds_
+// and, sorted ascending, into srt.  Bitonic network over P2 values held E per
+// thread (1024 threads; element E*t + r in thread t's register r): distances
+// below E are compare-exchanges in registers, partners inside the wave come
+// by lane shuffles, and only the 10 longest distances (of 78 passes at
+// 4096) exchange through LDS between workgroup barriers.  (DPP moves for the
+// partners up to 8 lanes away measured slower than the shuffles.)  srt has
+// room for max(P2, 1024) floats.
+template <int E>
+__device__ __forceinline__ void lag_sort(const float *xs, float *ds_, float *srt, int Csub, int lag,
+                                         int P2) {
+  const int t = threadIdx.x;
+  float v[E];
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    const int k = E * t + r;
+    float dv = INFINITY;
+    if (k < Csub) {
+      if (lag < 0) {
+        const int cut = Csub + lag;
+        dv = (k < cut) ? xs[k] - xs[k - lag] : xs[cut] - xs[Csub - 1];
+      } else {
+        dv = (k < lag) ? xs[lag] - xs[0] : xs[k] - xs[k - lag];
+      }
+      ds_[k] = dv;
+    }
+    v[r] = dv;
+  }
+  for (int kk = 2; kk <= P2; kk <<= 1) {
+    for (int j = kk >> 1; j > 0; j >>= 1) {
+      if (j < E) {
+        // compile-time distances keep v[] in registers
+        if (j == 1) sort_pass_regs<E, 1>(v, t, kk);
+        else if (j == 2) sort_pass_regs<E, 2>(v, t, kk);
+        else if (j == 4) sort_pass_regs<E, 4>(v, t, kk);
+        else sort_pass_regs<E, 8>(v, t, kk);
+        continue;
+      }
+      const int m = j / E;   // partner thread t ^ m holds the other element
+      float pv[E];
+      if (m < 64) {
+#pragma unroll
+        for (int r = 0; r < E; ++r) pv[r] = __shfl_xor(v[r], m);
+      } else {
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < E; ++r) srt[E * t + r] = v[r];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < E; ++r) pv[r] = srt[E * (t ^ m) + r];
+      }
+      const bool lower = (t & m) == 0;
+#pragma unroll
+      for (int r = 0; r < E; ++r) {
+        const bool asc = ((E * (lower ? t : (t ^ m)) + r) & kk) == 0;
+        const float mn = fminf(v[r], pv[r]), mx = fmaxf(v[r], pv[r]);
+        v[r] = (lower == asc) ? mn : mx;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < E; ++r) srt[E * t + r] = v[r];
+}
+
 // One workgroup per (window, lag): statistic from the window's segment sums
 // (src/core/iqrm.cu:72-91), lagged difference with edge clipping (:94-117),
-// quartiles by a bitonic sort in LDS and linear interpolation in double
+// quartiles by a bitonic sort (lag_sort) and linear interpolation in double
 // (:288-306), and the flag test (:120-135).
 template <typename T>
 __device__ __forceinline__ void lag_body(const LagArgs &a, uint32_t bid, uint32_t) {
@@ -281,49 +363,14 @@ __device__ __forceinline__ void lag_body(const LagArgs &a, uint32_t bid, uint32_
     xs[c] = xv;
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < P2; k += blockDim.x) {
-    float dv = INFINITY;
-    if (k < Csub) {
-      if (lag < 0) {
-        const int cut = Csub + lag;
-        dv = (k < cut) ? xs[k] - xs[k - lag] : xs[cut] - xs[Csub - 1];
-      } else {
-        dv = (k < lag) ? xs[lag] - xs[0] : xs[k] - xs[k - lag];
-      }
-      ds_[k] = dv;
-    }
-    srt[k] = dv;
-  }
-  __syncthreads();
-  // bitonic sort, ascending.  Thread t takes the pairs p = t + m*blockDim.
-  // A pass with distance j <= 64 keeps each wave inside its own 128-element
-  // blocks, so two such passes in a row need only the wave's own ordering;
-  // any pass next to a longer one is separated by a workgroup barrier.
-  const int half = P2 >> 1;
-  bool prev_local = false;
-  for (int kk = 2; kk <= P2; kk <<= 1) {
-    for (int j = kk >> 1; j > 0; j >>= 1) {
-      const bool local = j <= 64;
-      if (local && prev_local) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      } else {
-        __syncthreads();
-      }
-      prev_local = local;
-      const int lj = __builtin_ctz((unsigned)j);
-      for (int p = threadIdx.x; p < half; p += blockDim.x) {
-        const int i = ((p >> lj) << (lj + 1)) | (p & (j - 1));
-        const int q = i + j;
-        const bool asc = (i & kk) == 0;
-        float a = srt[i], b = srt[q];
-        if ((a > b) == asc) {
-          srt[i] = b;
-          srt[q] = a;
-        }
-      }
-    }
+  // lagged differences: ds_ (for the flags) and the sort input, E per
+  // thread (thread t holds elements E*t .. E*t+E-1; +inf past Csub)
+  switch (P2 <= 1024 ? 1 : P2 / 1024) {
+  case 1: lag_sort<1>(xs, ds_, srt, Csub, lag, P2); break;
+  case 2: lag_sort<2>(xs, ds_, srt, Csub, lag, P2); break;
+  case 4: lag_sort<4>(xs, ds_, srt, Csub, lag, P2); break;
+  case 8: lag_sort<8>(xs, ds_, srt, Csub, lag, P2); break;
+  default: lag_sort<16>(xs, ds_, srt, Csub, lag, P2); break;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -365,11 +412,15 @@ __device__ __forceinline__ int iqrm_partner(int k, int lag, int C) {
   return (k < lag) ? 0 : k - lag;
 }
 
-// Votes and final window mask (src/core/iqrm.cu:138-185).  The window's
-// flags are first gathered into one bit per lag per channel in LDS (eight
-// byte loads in flight per thread; nw = ceil(nlags / 32) words per channel),
-// then the casting counts and the test read only LDS.
-// LDS: Csub x (1 + nw) x 4 bytes.
+// Votes and final window mask (src/core/iqrm.cu:138-185), one workgroup per
+// (window, block of kMergeBlock channels).  A channel's mask needs the casting
+// counts of its partners (at most R = max |lag| away), and a count needs the
+// flags of channels at most R from it (the edge partners 0 and C-1 collect
+// channels within R of the edge), so the block's flags are gathered over a
+// halo of 2R: one bit per lag per channel in LDS (eight byte loads in flight
+// per thread; nw = ceil(nlags / 32) words per channel).  The counts over the
+// block and a halo of R then come from LDS atomics, and the test reads LDS.
+constexpr int kMergeBlock = 1024;
 __device__ __forceinline__ void merge_body(const MergeArgs &a, uint32_t bid, uint32_t) {
   const uint8_t *__restrict__ flags = a.flags;
   const int *__restrict__ lags = a.lags;
@@ -377,10 +428,20 @@ __device__ __forceinline__ void merge_body(const MergeArgs &a, uint32_t bid, uin
   const int nw = (nlags + 31) / 32;
   uint8_t *__restrict__ masks = a.masks;
   extern __shared__ int cast[];
-  uint32_t *bits = (uint32_t *)(cast + Csub);   // [nw][Csub]
-  const int w = (int)bid;
+  const int nblk = (Csub + kMergeBlock - 1) / kMergeBlock;
+  const int w = (int)bid / nblk, blk = (int)bid % nblk;
+  int R = 0;
+  for (int l = 0; l < nlags; ++l) {
+    const int al = abs(lags[l]);
+    if (al < Csub) R = max(R, al);
+  }
+  const int b0 = blk * kMergeBlock, b1 = min(b0 + kMergeBlock, Csub);
+  const int f0 = max(0, b0 - 2 * R), f1 = min(Csub, b1 + 2 * R);   // flags gathered
+  const int g0 = max(0, b0 - R), g1 = min(Csub, b1 + R);           // counts formed
+  uint32_t *bits = (uint32_t *)(cast + (g1 - g0));                 // [nw][f1 - f0]
+  const int nf = f1 - f0;
   const uint8_t *f = flags + (size_t)w * nlags * Csub;
-  for (int k = threadIdx.x; k < Csub; k += blockDim.x) {
+  for (int k = f0 + threadIdx.x; k < f1; k += blockDim.x) {
     for (int l0 = 0; l0 < nlags; l0 += 32) {
       const int l1 = min(l0 + 32, nlags);
       uint32_t b = 0;
@@ -393,27 +454,33 @@ __device__ __forceinline__ void merge_body(const MergeArgs &a, uint32_t bid, uin
         for (int u = 0; u < 8; ++u) b |= (v[u] ? 1u : 0u) << (l + u - l0);
       }
       for (; l < l1; ++l) b |= (f[(size_t)l * Csub + k] ? 1u : 0u) << (l - l0);
-      bits[(size_t)(l0 >> 5) * Csub + k] = b;
+      bits[(size_t)(l0 >> 5) * nf + (k - f0)] = b;
     }
-    cast[k] = 0;
   }
+  for (int j = g0 + threadIdx.x; j < g1; j += blockDim.x) cast[j - g0] = 0;
   __syncthreads();
-  for (int k = threadIdx.x; k < Csub; k += blockDim.x)
+  for (int k = f0 + threadIdx.x; k < f1; k += blockDim.x)
     for (int l = 0; l < nlags; ++l)
-      if ((bits[(size_t)(l >> 5) * Csub + k] >> (l & 31)) & 1u)
-        atomicAdd(&cast[iqrm_partner(k, lags[l], Csub)], 1);
+      if ((bits[(size_t)(l >> 5) * nf + (k - f0)] >> (l & 31)) & 1u) {
+        const int j = iqrm_partner(k, lags[l], Csub);
+        if (j >= g0 && j < g1) atomicAdd(&cast[j - g0], 1);
+      }
   __syncthreads();
   uint8_t *mrow = masks + (size_t)w * nchan;
-  for (int c = threadIdx.x; c < nchan; c += blockDim.x) {
+  // channels outside the IQRM range are 0: the first block writes those
+  // below it, the last block those above it
+  const int cl = blk == 0 ? 0 : c0 + b0;
+  const int ch = blk == nblk - 1 ? nchan : c0 + b1;
+  for (int c = cl + threadIdx.x; c < ch; c += blockDim.x) {
     const int k = c - c0;
     uint8_t m = 0;
     if (k >= 0 && k < Csub) {
       int recv = 0;
-      for (int q = 0; q < nw; ++q) recv += __builtin_popcount(bits[(size_t)q * Csub + k]);
+      for (int q = 0; q < nw; ++q) recv += __builtin_popcount(bits[(size_t)q * nf + (k - f0)]);
       if (recv > 0)
         for (int l = 0; l < nlags; ++l)
-          if (((bits[(size_t)(l >> 5) * Csub + k] >> (l & 31)) & 1u) &&
-              cast[iqrm_partner(k, lags[l], Csub)] < recv) {
+          if (((bits[(size_t)(l >> 5) * nf + (k - f0)] >> (l & 31)) & 1u) &&
+              cast[iqrm_partner(k, lags[l], Csub) - g0] < recv) {
             m = 1;
             break;
           }
@@ -1257,10 +1324,17 @@ hipError_t launch_iqrm_lags(int dtype, const Part *segsum, const Seg *segs,
 }
 
 hipError_t launch_iqrm_merge(const uint8_t *flags, const int *lags, int nlags, int nwin,
-                             int Csub, int c0, int nchan, uint8_t *masks, hipStream_t st) {
+                             int Csub, int c0, int nchan, uint8_t *masks, int reach,
+                             hipStream_t st) {
   MergeArgs a{flags, lags, masks, nlags, Csub, c0, nchan};
-  const size_t lds = (size_t)Csub * (1 + (nlags + 31) / 32) * sizeof(int);
-  return emit(KB_MERGE, 4, (size_t)nwin, 1024, lds, a, st);
+  // LDS: counts over the block + R each side, bits over the block + 2R
+  // (R = reach, the largest |lag| below Csub)
+  const size_t R = (size_t)std::max(0, reach);
+  const size_t ncnt = std::min<size_t>(Csub, kMergeBlock + 2 * R);
+  const size_t nbit = std::min<size_t>(Csub, kMergeBlock + 4 * R);
+  const size_t lds = (ncnt + nbit * ((nlags + 31) / 32)) * sizeof(int);
+  const size_t nblk = (Csub + kMergeBlock - 1) / kMergeBlock;
+  return emit(KB_MERGE, 4, (size_t)nwin * nblk, 1024, lds, a, st);
 }
 
 hipError_t launch_segmask(const uint8_t *masks, const Seg *segs, int nseg, int nchan,

@@ -409,13 +409,15 @@ static af_status build(af_plan *p) {
     const int radius = std::max(1, (int)std::floor(rf * (float)p->Csub));
     p->lags = make_lags(radius, gf);
     p->nlags = (int)p->lags.size();
+    for (int l : p->lags)
+      if (std::abs(l) < p->Csub) p->lag_reach = std::max(p->lag_reach, std::abs(l));
     for (int w = 0; w < p->nwin; ++w) {
       bounds.push_back(p->wt0[w]);
       bounds.push_back(p->wt1[w]);
     }
     int P2 = 1;
     while (P2 < p->Csub) P2 <<= 1;
-    p->iqrm_lds = (size_t)(2 * p->Csub + P2) * sizeof(float);
+    p->iqrm_lds = (size_t)(2 * p->Csub + std::max(P2, 1024)) * sizeof(float);
     if (p->iqrm_lds > (size_t)maxlds)
       return fail(AF_ERR_INVALID, "IQRM channel range too large for one workgroup");
     // vote merge: casting counts + one bit per lag per channel
@@ -690,7 +692,7 @@ af_status run_iqrm_vote(af_plan *p, hipStream_t st) {
                             std::max(0.0f, d.iqrm_nsigma), p->d_flags, p->iqrm_lds, st));
   }
   AF_HIP(launch_iqrm_merge(p->d_flags, p->d_lags, p->nlags, p->nwin, p->Csub, d.iqrm_c0,
-                           d.nchan, p->d_masks, st));
+                           d.nchan, p->d_masks, p->lag_reach, st));
   AF_HIP(launch_segmask(p->d_masks, p->d_segs, p->nseg, d.nchan, p->d_segmask, st));
   return AF_OK;
 }

@@ -105,6 +105,25 @@ def test_iqrm_many_lags(gpu_lib):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("radius_frac,W", [(0.2, 0), (0.45, 700)])
+def test_iqrm_vote_blocks_with_wide_halo(gpu_lib, radius_frac, W):
+    # 3000 IQRM channels: the vote runs in three 1024-channel blocks whose
+    # lags reach 600 and 1350 channels, so each block gathers flags and
+    # casting counts far across its neighbours (and across the range edges)
+    say(f"case: IQRM vote blocks, 3000 ch, radius_frac {radius_frac}, W {W}")
+    rng = np.random.default_rng(12)
+    b = small_band(3001, 1100.0, 1500.0, 1e-4)
+    x = noise(rng, 2000, 3001)
+    synth.inject_narrowband_rfi(x, rng, frac=0.06, amp=40, n_stuck=6, pmax=300)
+    spec = make_spec(x, b, 0.0, 10.0, 8, method="direct", use_iqrm=True, iqrm_c0=0,
+                     iqrm_c1=3000, iqrm_mode=1, iqrm_radius_frac=radius_frac,
+                     iqrm_nsigma=2.5, iqrm_W=W, iqrm_H=W // 2)
+    got, ref, masks, rmask = run_both(x, spec)
+    assert masks.sum() > 0
+    assert np.array_equal(masks, rmask)
+    assert np.array_equal(got, ref)
+
+
 @pytest.mark.parametrize("ds", [2, 3])
 def test_subband_u8_downsampled(gpu_lib, ds):
     say(f"case: subband u8 ds={ds}")
