@@ -71,6 +71,12 @@ void release(af_batch *b) {
 
 using namespace af;
 
+namespace af {
+// workgroups from which one launch alone keeps every CU busy (256 CUs x a
+// few resident workgroups): such launches are not merged
+constexpr uint32_t kBatchFillGrid = 2048;
+}  // namespace af
+
 extern "C" {
 
 af_status af_batch_create(af_plan *const *plans, int nplans, const void *const *d_ins,
@@ -133,13 +139,27 @@ af_status af_batch_create(af_plan *const *plans, int nplans, const void *const *
             !(tiled[i] && (stage == 7 || stage == 8)))
           group.push_back(&d);
     if (group.empty()) return AF_OK;
-    if (!batchable(kind) || group.size() < 2) {
-      for (auto *d : group) {
+    // a launch that fills the machine by itself runs alone: merging it only
+    // adds the table lookup to each of its workgroups (measured ~3% slower on
+    // large chunks); the small launches of a stage are merged
+    std::vector<const LaunchDesc *> small;
+    for (auto *d : group) {
+      if (!batchable(kind) || d->nblk >= kBatchFillGrid) {
         Merged m;
         m.kind = kind;
         m.single = *d;
         b->launches.push_back(m);
+      } else {
+        small.push_back(d);
       }
+    }
+    group.swap(small);
+    if (group.empty()) return AF_OK;
+    if (group.size() < 2) {
+      Merged m;
+      m.kind = kind;
+      m.single = *group[0];
+      b->launches.push_back(m);
       return AF_OK;
     }
     Merged m;

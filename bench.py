@@ -64,6 +64,8 @@ def parse():
     ap.add_argument("--served-full-rfi", action="store_true",
                     help="served leg without incremental RFI (every window cleaned whole)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
+    ap.add_argument("--no-direct", action="store_true",
+                    help="skip the direct-method leg (1024 ch x 2^18, DM 0-1000)")
     ap.add_argument("--batch-only", action="store_true", help="run the batched leg alone")
     ap.add_argument("--served-only", action="store_true",
                     help="run the served leg alone (no plain-call line)")
@@ -354,6 +356,118 @@ def batch_leg(threads, budget_gb=110.0):
     }
 
 
+def direct_leg(threads, cpu_target_s, reps=10, with_cpu=True):
+    """The plain direct-method case, on the GPU and with the reference's CPU
+    loop nest on the same input: 1024 ch u8 x 2^18 samples over 1000-1500
+    MHz, one dispersed pulse (DM 500), DM 0-1000 step 1 (1001 trials),
+    zero-DM, full-length u32 plane."""
+    from astroflow_amd import capi, synth
+    x, band, _ = synth.direct_1024(seed=1, threads=threads)
+    nsamp, nchan = x.shape
+    f = band.freq
+    spec = capi.DedispSpec(dtype=np.uint8, nchan=nchan, nsamp=nsamp, freq=f,
+                           tsamp=float(band.tsamp), dm_low=0.0, dm_step=1.0, ndm=1001,
+                           dd_c0=0, dd_c1=nchan, f_ref=float(f[-1]), method="direct",
+                           use_zero_dm=True, zdm_c0=0, zdm_c1=nchan - 1)
+    plan = capi.Plan(spec)
+    d_in = capi.DeviceBuffer(x.nbytes)
+    d_in.upload(x)
+    d_out = capi.DeviceBuffer(1001 * nsamp * 4)
+    plan.execute(d_in.ptr, d_out.ptr)
+    capi.sync()
+    plan.set_profiling(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plan.execute(d_in.ptr, d_out.ptr)
+    capi.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    st = plan.stage_times()
+    plan.close()
+    d_in.free()
+    d_out.free()
+    work = 1001 * nsamp
+    res = {"workload": "1024 ch u8 1000-1500 MHz x 2^18 samples, direct method, DM 0-1000 "
+                       "step 1 (1001 trials), zero-DM, full-length u32 plane",
+           "ms": ms, "rate": work / (ms / 1e3), "dedisp_ms": st["dedisp"],
+           "adds_per_s": 1001.0 * nchan * nsamp / (st["dedisp"] / 1e3)}
+    if with_cpu:
+        from oracle import oracle as O   # CPU baseline leg: the oracle's cpucal restatement
+        # cpucal's channel range excludes the top channel: [0, nchan - 1)
+        all_dms = np.arange(1001, dtype=np.float64)
+
+        def run(k):
+            idx = np.linspace(0, 1000, k).round().astype(np.int64)
+            d = O.delay_table(f, 0, nchan - 1, f[-1], float(band.tsamp), 1, all_dms[idx])
+            t = time.perf_counter()
+            O.cpucal_direct_u8(x, 0, nchan - 1, d, 0, nsamp, 1, nthreads=threads)
+            return time.perf_counter() - t
+
+        dt = run(4)
+        k = int(min(1001, max(4, round(4 * cpu_target_s / max(dt, 1e-3)))))
+        el = run(k)
+        cpu_rate = k * nsamp / el
+        res["cpu_baseline"] = {
+            "value": cpu_rate, "unit": UNIT, "cores": threads, "kind": "port",
+            "sample": f"{k} of 1001 DM trials (evenly spaced) x {nsamp} samples x {nchan - 1} "
+                      f"channels, cpucal direct loop nest (include/cpucal.hpp:167-186), "
+                      f"{el:.2f} s"}
+        res["speedup_vs_cpu"] = res["rate"] / cpu_rate
+    return res
+
+
+def small_batch_leg(threads, nchunk=64, reps=5):
+    """Many small beam-chunks in one batch, where merging pays: 64 chunks of
+    1024 ch u8 x 16384 samples, DM 0-500 step 1 (501 trials), 32-channel
+    subbands, IQRM + zero-DM.  Plain executes one after another vs the
+    batched launches vs their hipGraph replay (inputs resident in HBM)."""
+    from astroflow_amd import capi
+    c = dict(nchan=1024, nsamp=16384, dtype=np.uint8, dm_max=500.0, dm_step=1.0, ds=1)
+    spec = c3_spec(c)
+    from astroflow_amd import synth
+    synth_pool = synth.noise_u8_threaded(43, 16384, 1024, threads=threads)
+    plans, ins, outs = [], [], []
+    for _ in range(nchunk):
+        p = capi.Plan(spec)
+        di = capi.DeviceBuffer(synth_pool.nbytes)
+        di.upload(synth_pool)
+        do = capi.DeviceBuffer(p.info.ndm * p.info.nout * 4)
+        plans.append(p)
+        ins.append(di)
+        outs.append(do)
+    capi.sync()
+
+    def plain():
+        for p, di, do in zip(plans, ins, outs):
+            p.execute(di.ptr, do.ptr)
+
+    batch = capi.Batch(plans, [b.ptr for b in ins], [b.ptr for b in outs])
+    inf = batch.info()
+    res = {}
+    for name, fn in (("plain", plain), ("merged", lambda: batch.execute(use_graph=False)),
+                     ("graph", lambda: batch.execute(use_graph=True))):
+        fn()
+        capi.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        capi.sync()
+        res[name + "_ms"] = (time.perf_counter() - t0) * 1e3 / reps
+    batch.close()
+    for p in plans:
+        p.close()
+    for b in ins + outs:
+        b.free()
+    work = nchunk * 501 * 16384
+    return {"workload": f"{nchunk} chunks of 1024 ch u8 x 16384 samples, DM 0-500 step 1 "
+                        "(501 trials), 32-channel subbands, IQRM + zero-DM",
+            "launches_batched": inf["launches"], "launches_plain": inf["recorded"],
+            **res,
+            "plain_rate": work / (res["plain_ms"] / 1e3),
+            "graph_rate": work / (res["graph_ms"] / 1e3),
+            "speedup_merged_vs_plain": res["plain_ms"] / res["merged_ms"],
+            "speedup_graph_vs_plain": res["plain_ms"] / res["graph_ms"]}
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -375,7 +489,7 @@ def main():
         print(json.dumps({"served": served_leg(threads, a.served_chunks, a.served_full_rfi)}))
         return
     if a.batch_only:
-        print(json.dumps({"batched": batch_leg(threads)}))
+        print(json.dumps({"batched": batch_leg(threads), "batched_small": small_batch_leg(threads)}))
         return
     x, spec, info = make_workload(threads)
     spec.dm_per_block = a.dm_per_block
@@ -459,6 +573,11 @@ def main():
         res["served"] = served_leg(threads, a.served_chunks, a.served_full_rfi)
     if not a.no_batch:
         res["batched"] = batch_leg(threads)
+        res["batched_small"] = small_batch_leg(threads)
+        log(f"small-chunk batch: {json.dumps(res['batched_small'])}")
+    if not a.no_direct:
+        res["direct_1024"] = direct_leg(threads, a.cpu_seconds / 2, with_cpu=not a.no_cpu)
+        log(f"direct leg: {res['direct_1024']['ms']:.2f} ms per call")
     if not a.no_cpu:
         res["cpu_baseline"] = cpu_baseline(x, info, a.cpu_seconds)
         res["speedup_vs_cpu"] = value / res["cpu_baseline"]["value"]

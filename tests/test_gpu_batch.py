@@ -1,5 +1,6 @@
 """Batched execution: plans of different nchan / nsamp / DM range / dtype /
-method run as one launch per stage (and replayed from a hipGraph); every
+method run with one merged launch per stage for their small launches (a
+launch that fills the GPU runs alone), replayed from a hipGraph; every
 output equals the plan's own execute bit for bit; the autotuner picks only
 result-invariant settings unless asked to tune the subband width."""
 import numpy as np
@@ -46,6 +47,12 @@ def batch_cases():
     cases.append((x, make_spec(x, b4, 0.0, 1.0, 150, method="subband", sbw=64, ndm0=16,
                                nout=17000, use_zero_dm=True, zdm_c0=0, zdm_c1=384,
                                inter_budget=10 * 6 * 2 * 4096)))
+    # a plan whose stage-1 launch fills the GPU on its own (4096 channels in
+    # 32-channel subbands: over 2048 workgroups), so it runs unmerged
+    b5 = small_band(4096, 1000.0, 1500.0, 1e-4)
+    x = noise(rng, 12000, 4096)
+    cases.append((x, make_spec(x, b5, 0.0, 1.0, 256, method="subband", sbw=32, ndm0=16,
+                               use_zero_dm=True, zdm_c0=0, zdm_c1=4095)))
     # a second plan of the first shape (same kernels, merged with it)
     x = noise(rng, 6000, 256)
     cases.append((x, make_spec(x, b1, 0.0, 1.0, 90, method="subband", sbw=32, ndm0=16,
