@@ -128,6 +128,7 @@ struct DdArgs {
   const int4 *worklist;     // optional (grp, tile, rowblk, -) per workgroup, in launch order
   int nwork;
   int u8q;                  // u8: the aligned-load kernel (kDdRowsQ rows in step, kDdTileQ tiles)
+  int f32q;                 // float32: kDdRowsQ rows in step, kDdTile tiles (k_dd_f32_q)
 };
 hipError_t launch_dedisperse(int dtype, const DdArgs &a, hipStream_t st);
 
@@ -212,7 +213,7 @@ enum LaunchKind : int {
   KB_LAG_U8, KB_LAG_U16, KB_LAG_U32, KB_LAG_F32,
   KB_MERGE, KB_SEGMASK,
   KB_CLEAN_U8, KB_CLEAN_U16, KB_CLEAN_U32, KB_CLEAN_F32,
-  KB_DD_U8Q, KB_DD_U8, KB_DD_U8LDS, KB_DD_G_U8, KB_DD_G_U16, KB_DD_G_U32, KB_DD_G_F32F,
+  KB_DD_U8Q, KB_DD_F32Q, KB_DD_U8, KB_DD_U8LDS, KB_DD_G_U8, KB_DD_G_U16, KB_DD_G_U32, KB_DD_G_F32F,
   KB_DD_G_F32D,
   KB_S2_V4, KB_S2_V8, KB_S2_V16, KB_S2_C8, KB_S2_C16, KB_S2_U16, KB_S2_G_U16, KB_S2_G_U32, KB_S2_G_U64, KB_S2_G_F32,
   KB_S2_F32C8, KB_S2_F32C16,
@@ -279,6 +280,7 @@ struct af_plan {
   int s2_pk = 0;            // u16 elements of LDS per workgroup, packed rows (0: uniform rows)
   int dd_lds_rows = 0, dd_lds_win = 0;   // LDS-staged u8 dedispersion
   bool dd_q = false;                     // aligned-load u8 dedispersion (kDdTileQ tiles)
+  bool dd_fq = false;                    // float32 rows-in-step dedispersion (kDdTile tiles)
   std::vector<int4> worklist;            // stage-1 workgroups in locality order
   int4 *d_worklist = nullptr;
   double adds = 0;

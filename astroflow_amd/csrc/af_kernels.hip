@@ -776,6 +776,115 @@ __global__ __launch_bounds__(64 * kDdRowsQ) void k_dd_u8_q(DdArgs a) {
   else dd_u8_q_body<false>(a, lid, gridDim.x);
 }
 
+// float32 dedispersion (direct, or subband partial sums): a workgroup of
+// kDdRowsQ waves owns kDdRowsQ consecutive DM rows of one channel group and
+// one tile of kDdTile outputs, the rows walking the group's channels in step
+// (a barrier every 8 channels, so overlapping windows are fetched into the L1
+// once, as in k_dd_u8_q).  Load instruction i (0..3) of lane l reads the 4
+// floats at 256 i + 4 l of the row's window, so each instruction reads 1 KB
+// contiguous; sums in FP64, stored as float32.  Work list order and XCD
+// placement as for u8.
+template <bool O32>
+__device__ __forceinline__ void dd_f32_q_body(const DdArgs &a, uint32_t lid, uint32_t) {
+  const int nrowblk = (a.nrows + kDdRowsQ - 1) / kDdRowsQ;
+  int rowblk, tile, grp;
+  if (a.worklist) {
+    const int4 wi = a.worklist[lid];
+    grp = __builtin_amdgcn_readfirstlane(wi.x);
+    tile = __builtin_amdgcn_readfirstlane(wi.y);
+    rowblk = __builtin_amdgcn_readfirstlane(wi.z);
+  } else {
+    rowblk = (int)(lid % (uint32_t)nrowblk);
+    const uint32_t rest = lid / (uint32_t)nrowblk;
+    tile = (int)(rest % (uint32_t)a.ntiles);
+    grp = (int)(rest / (uint32_t)a.ntiles);
+  }
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  // every wave runs to the end (the rows meet at barriers): rows past the
+  // last compute row nrows - 1 again and store nothing
+  const int row_w = rowblk * kDdRowsQ + wave;
+  const bool row_ok = row_w < a.nrows;
+  const int row = row_ok ? row_w : a.nrows - 1;
+  const int ca = grp * a.grp_w;
+  const int cb = min(ca + a.grp_w, a.nch);
+  const float *__restrict__ tb = (const float *)a.xT + (size_t)tile * kDdTile;
+  typedef __attribute__((address_space(4))) const int32_t cint32;
+  cint32 *dc = (cint32 *)(a.delays + (size_t)row * a.nch);
+  const __amdgpu_buffer_rsrc_t rsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void *)tb, (short)0, -1, 0x00020000);
+  const uint32_t loff = (uint32_t)lane * 16u;   // bytes
+  auto load = [&](int c, int d, int i) {
+    if (O32) {
+      const uint32_t off = ((uint32_t)c * (uint32_t)a.Lp + (uint32_t)d + 256u * (uint32_t)i) * 4u;
+      const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)loff, (int)off, 0);
+      return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+                         __uint_as_float(v[3]));
+    } else {
+      float4 v;
+      __builtin_memcpy(&v, tb + (size_t)c * a.Lp + d + 256 * i + 4 * lane, 16);
+      return v;
+    }
+  };
+  double acc[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) acc[i] = 0.0;
+  auto add = [&](const float4 (&v)[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      acc[4 * i + 0] += (double)v[i].x;
+      acc[4 * i + 1] += (double)v[i].y;
+      acc[4 * i + 2] += (double)v[i].z;
+      acc[4 * i + 3] += (double)v[i].w;
+    }
+  };
+  int c = ca;
+  for (; c + 8 <= cb; c += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; u += 2) {
+      float4 v0[4], v1[4];
+      const int d0 = dc[c + u], d1 = dc[c + u + 1];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        v0[i] = load(c + u, d0, i);
+        v1[i] = load(c + u + 1, d1, i);
+      }
+      add(v0);
+      add(v1);
+    }
+    __builtin_amdgcn_s_barrier();
+  }
+  for (; c < cb; ++c) {
+    float4 v[4];
+    const int d = dc[c];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = load(c, d, i);
+    add(v);
+  }
+  if (!row_ok) return;
+  const size_t t0 = (size_t)tile * kDdTile;
+  float *o = (float *)a.out + ((size_t)row * a.ngrp + grp) * a.ostride + t0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const size_t t = (size_t)256 * i + 4 * (size_t)lane;
+    const float4 w = make_float4((float)acc[4 * i], (float)acc[4 * i + 1], (float)acc[4 * i + 2],
+                                 (float)acc[4 * i + 3]);
+    if (t0 + t + 4 <= a.nvalid) {
+      __builtin_memcpy(o + t, &w, 16);
+    } else {
+      const float wv[4] = {w.x, w.y, w.z, w.w};
+      for (int k = 0; k < 4; ++k)
+        if (t0 + t + k < a.nvalid) o[t + k] = wv[k];
+    }
+  }
+}
+
+__global__ __launch_bounds__(64 * kDdRowsQ) void k_dd_f32_q(DdArgs a) {
+  const uint32_t lid = xcd_remap(blockIdx.x, gridDim.x);
+  if ((uint64_t)a.nch * a.Lp * 4 <= 0xFFFFFFFFull) dd_f32_q_body<true>(a, lid, gridDim.x);
+  else dd_f32_q_body<false>(a, lid, gridDim.x);
+}
+
 __device__ __forceinline__ void dd_u8_body(const DdArgs &a, uint32_t lid, uint32_t) {
   const int nrowblk = (a.nrows + kDdRowsPerBlock - 1) / kDdRowsPerBlock;
   int rowblk, tile, grp;
@@ -1236,6 +1345,9 @@ AF_BATCHED(kb_clean_f32, CleanArgs, 256, (clean_body<float>(a, lid, nb)))
 AF_BATCHED_REF(kb_dd_u8q, DdArgs, 64 * kDdRowsQ,
                 ((uint64_t)a.nch * a.Lp <= 0xFFFFFFFFull ? dd_u8_q_body<true>(a, lid, nb)
                                                          : dd_u8_q_body<false>(a, lid, nb)))
+AF_BATCHED_REF(kb_dd_f32q, DdArgs, 64 * kDdRowsQ,
+                ((uint64_t)a.nch * a.Lp * 4 <= 0xFFFFFFFFull ? dd_f32_q_body<true>(a, lid, nb)
+                                                             : dd_f32_q_body<false>(a, lid, nb)))
 AF_BATCHED(kb_dd_u8, DdArgs, 256, (dd_u8_body(a, lid, nb)))
 AF_BATCHED(kb_dd_g_u8, DdArgs, 256, (dd_generic_body<uint8_t, uint32_t, uint32_t>(a, lid, nb)))
 AF_BATCHED(kb_dd_g_u16, DdArgs, 256, (dd_generic_body<uint16_t, uint32_t, uint32_t>(a, lid, nb)))
@@ -1369,7 +1481,11 @@ hipError_t launch_dedisperse(int dtype, const DdArgs &a, hipStream_t st) {
   case AF_U32: return emit(KB_DD_G_U32, 7, nblk, 256, 0, a, st);
   // float32: FP64 accumulation, float32 stores (the output, or the subband
   // partial sums)
-  case AF_F32: return emit(KB_DD_G_F32F, 7, nblk, 256, 0, a, st);
+  case AF_F32:
+    if (a.f32q)
+      return emit(KB_DD_F32Q, 7, (size_t)((a.nrows + kDdRowsQ - 1) / kDdRowsQ) * a.ntiles * a.ngrp,
+                  64 * kDdRowsQ, 0, a, st);
+    return emit(KB_DD_G_F32F, 7, nblk, 256, 0, a, st);
   default: return hipErrorInvalidValue;
   }
 }
@@ -1488,6 +1604,7 @@ hipError_t launch_desc(const LaunchDesc &d, hipStream_t st) {
   case KB_CLEAN_U32: hipLaunchKernelGGL(k_clean<uint32_t>, g, b, d.lds, st, argsof<CleanArgs>(d)); break;
   case KB_CLEAN_F32: hipLaunchKernelGGL(k_clean<float>, g, b, d.lds, st, argsof<CleanArgs>(d)); break;
   case KB_DD_U8Q: hipLaunchKernelGGL(k_dd_u8_q, g, b, d.lds, st, argsof<DdArgs>(d)); break;
+  case KB_DD_F32Q: hipLaunchKernelGGL(k_dd_f32_q, g, b, d.lds, st, argsof<DdArgs>(d)); break;
   case KB_DD_U8: hipLaunchKernelGGL(k_dd_u8, g, b, d.lds, st, argsof<DdArgs>(d)); break;
   case KB_DD_U8LDS: hipLaunchKernelGGL(k_dd_u8_lds<16>, g, b, d.lds, st, argsof<DdArgs>(d)); break;
   case KB_DD_G_U8:
@@ -1582,6 +1699,7 @@ hipError_t launch_batched(int kind, const void *tab, const uint32_t *start, int 
     AF_BL(KB_CLEAN_U32, kb_clean_u32, 256, CleanArgs)
     AF_BL(KB_CLEAN_F32, kb_clean_f32, 256, CleanArgs)
     AF_BL(KB_DD_U8Q, kb_dd_u8q, 64 * kDdRowsQ, DdArgs)
+    AF_BL(KB_DD_F32Q, kb_dd_f32q, 64 * kDdRowsQ, DdArgs)
     AF_BL(KB_DD_U8, kb_dd_u8, 256, DdArgs)
     AF_BL(KB_DD_G_U8, kb_dd_g_u8, 256, DdArgs)
     AF_BL(KB_DD_G_U16, kb_dd_g_u16, 256, DdArgs)

@@ -356,6 +356,7 @@ static af_status build(af_plan *p) {
   // the aligned-load u8 kernel (channel lists sorted by delay quarter class)
   p->dd_q = d.dtype == AF_U8 && p->dd_lds_rows == 0 &&
             (d.method == AF_METHOD_DIRECT || d.sb_nchan <= 256);
+  p->dd_fq = d.dtype == AF_F32;
   const int dd_tile = p->dd_q ? kDdTileQ : kDdTile;
   p->ntiles1 = (int)((n_ext + dd_tile - 1) / dd_tile);
   p->L1 = (size_t)p->ntiles1 * dd_tile;
@@ -500,9 +501,10 @@ static af_status build(af_plan *p) {
   // workgroups that run together on one XCD (consecutive launch order after
   // the XCD remap) read overlapping input, so it is fetched from HBM about
   // once instead of once per tile.
-  if (d.dtype == AF_U8) {
+  if (d.dtype == AF_U8 || p->dd_fq) {
     const int rows = d.method == AF_METHOD_DIRECT ? p->ndm : p->ndm_nom;
-    const int JB = p->dd_lds_rows ? p->dd_lds_rows : p->dd_q ? kDdRowsQ : kDdRowsPerBlock;
+    const int JB = p->dd_lds_rows ? p->dd_lds_rows
+                                  : (p->dd_q || p->dd_fq) ? kDdRowsQ : kDdRowsPerBlock;
     const int nrb = (rows + JB - 1) / JB;
     const int ngrp = d.method == AF_METHOD_DIRECT ? 1 : p->nsb;
     const int gw = d.method == AF_METHOD_DIRECT ? p->nch : d.sb_nchan;
@@ -711,6 +713,7 @@ af_status run_dedisp(af_plan *p, const void *xT, size_t Lp, void *d_out, hipStre
   da.lds_win = p->dd_lds_win;
   da.worklist = p->d_worklist;
   da.u8q = p->dd_q ? 1 : 0;
+  da.f32q = p->dd_fq ? 1 : 0;
   da.nwork = (int)p->worklist.size();
   if (d.method == AF_METHOD_DIRECT) {
     da.nrows = p->ndm;
