@@ -629,7 +629,7 @@ template <bool O32, bool SYNC>
 __device__ __forceinline__ void dd_q_channels(const uint8_t *__restrict__ tb, size_t Lp,
                                               uint32_t loff, const int32_t *__restrict__ drow,
                                               int c0, int c1, uint32_t (&S)[4],
-                                              uint32_t (&H)[4]) {
+                                              uint32_t (&H)[4], bool live = true) {
   const __amdgpu_buffer_rsrc_t rsrc =
       __builtin_amdgcn_make_buffer_rsrc((void *)tb, (short)0, -1, 0x00020000);
   auto load = [&](int c, int d, uint4 &lo) {
@@ -659,6 +659,12 @@ __device__ __forceinline__ void dd_q_channels(const uint8_t *__restrict__ tb, si
   typedef __attribute__((address_space(4))) const int32_t cint32;
   cint32 *dc = (cint32 *)drow;
   int c = c0;
+  // a wave past the last DM row (live false, wave-uniform) only meets the
+  // others at the barriers
+  if (SYNC && !live) {
+    for (; c + 8 <= c1; c += 8) __builtin_amdgcn_s_barrier();
+    return;
+  }
   for (; c + 8 <= c1; c += 8) {
     uint4 lo[8];
     uint32_t sh[8];
@@ -718,7 +724,7 @@ __device__ __forceinline__ void dd_u8_q_body(const DdArgs &a, uint32_t lid, uint
   if (cb - ca <= blk_w) {
     // one block (a subband): the totals are not live during the channel loop
     uint32_t S[4] = {0, 0, 0, 0}, H[4] = {0, 0, 0, 0};
-    dd_q_channels<O32, true>(tb, a.Lp, loff, drow, ca, cb, S, H);
+    dd_q_channels<O32, true>(tb, a.Lp, loff, drow, ca, cb, S, H, row_ok);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const uint32_t L = S[k] - (H[k] << 8);
@@ -732,7 +738,7 @@ __device__ __forceinline__ void dd_u8_q_body(const DdArgs &a, uint32_t lid, uint
     for (int i = 0; i < 16; ++i) tot[i] = 0;
     for (int b0 = ca; b0 < cb; b0 += blk_w) {
       uint32_t S[4] = {0, 0, 0, 0}, H[4] = {0, 0, 0, 0};
-      dd_q_channels<O32, true>(tb, a.Lp, loff, drow, b0, min(b0 + blk_w, cb), S, H);
+      dd_q_channels<O32, true>(tb, a.Lp, loff, drow, b0, min(b0 + blk_w, cb), S, H, row_ok);
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const uint32_t L = S[k] - (H[k] << 8);
@@ -839,6 +845,11 @@ __device__ __forceinline__ void dd_f32_q_body(const DdArgs &a, uint32_t lid, uin
     }
   };
   int c = ca;
+  if (!row_ok) {
+    // past the last DM row: only meet the other rows at the barriers
+    for (; c + 8 <= cb; c += 8) __builtin_amdgcn_s_barrier();
+    return;
+  }
   for (; c + 8 <= cb; c += 8) {
 #pragma unroll
     for (int u = 0; u < 8; u += 2) {
