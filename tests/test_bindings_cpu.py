@@ -77,10 +77,10 @@ def test_filterbank_reader_16bit_ascending(tmp_path):
     assert f.nbits == 16 and np.array_equal(f.data[:, 0, :], x)
 
 
-REF_FIL = "/root/reference/tests/FRB180417.fil"
+# the reference's own test fixture (tests/FRB180417.fil), kept unchanged under tests/golden/
+REF_FIL = os.path.join(os.path.dirname(__file__), "golden", "FRB180417.fil")
 
 
-@pytest.mark.skipif(not os.path.exists(REF_FIL), reason="reference fixture not present")
 def test_reads_the_reference_fixture_file():
     # tests/FRB180417.fil of the reference: 336 channels, 8-bit, 5120 samples
     f = core.Filterbank(REF_FIL)
