@@ -1,7 +1,9 @@
 """The HIP path on real data: the reference's fixture tests/FRB180417.fil
 (336 ch, 8-bit, 5120 samples; FRB 180417, published DM 474.8).  Planes and
 IQRM masks equal the oracle's bit for bit, for both methods, and the
-drop-in `_dedisperse_spec_raw` finds the burst at its DM."""
+drop-in `_dedisperse_spec_raw` finds the burst at its DM.  The float32 path
+(the same samples as float32) is within 1e-6 * sum|x| of the FP64 oracle, and
+2x time binning is bit-exact."""
 import sys
 
 import numpy as np
@@ -39,6 +41,47 @@ def test_real_data_planes_and_masks_equal_the_oracle(gpu_lib, method):
     assert np.array_equal(got, ref)
     m, snr, t = R.best_dm(got)
     assert abs(m - R.FRB_DM) <= 2.5 and abs(t - 1602) <= 3
+
+
+def _spec(freq, tsamp, method, dtype=np.uint8, ds=1):
+    return capi.DedispSpec(dtype=dtype, nchan=336, nsamp=5120, freq=freq, tsamp=tsamp,
+                           dm_low=0.0, dm_step=1.0, ndm=1001, dd_c0=0, dd_c1=336,
+                           f_ref=float(freq[-1]), method=method, sbw=16, ndm0=8, ds=ds,
+                           use_iqrm=True, iqrm_c0=0, iqrm_c1=335, use_zero_dm=True, zdm_c0=0,
+                           zdm_c1=336)
+
+
+def test_real_data_float32_within_tolerance(gpu_lib):
+    say("case: FRB180417.fil as float32, subband, IQRM + zero-DM")
+    from tests.test_gpu_parity import assert_f32_close
+    from tests.helpers import oracle_spec
+    x, freq, tsamp, _ = R.load()
+    xf = x.astype(np.float32)
+    spec = _spec(freq, tsamp, "subband", np.float32)
+    plan = capi.Plan(spec)
+    got, masks = plan.run_host(xf, want_masks=True)
+    plan.close()
+    ref, rmask, *_ = O.run_pipeline(xf, oracle_spec(spec), O.IQRM_GPU)
+    assert np.array_equal(masks, rmask)
+    assert_f32_close(got, ref, xf, spec)
+    m, snr, t = R.best_dm(got)
+    assert abs(m - R.FRB_DM) <= 2.5 and abs(t - 1602) <= 3
+
+
+def test_real_data_downsampled_by_two(gpu_lib):
+    say("case: FRB180417.fil, direct, ds 2")
+    from tests.helpers import oracle_spec
+    x, freq, tsamp, _ = R.load()
+    spec = _spec(freq, tsamp, "direct", ds=2)
+    plan = capi.Plan(spec)
+    got, masks = plan.run_host(x, want_masks=True)
+    plan.close()
+    ref, rmask, *_ = O.run_pipeline(x, oracle_spec(spec), O.IQRM_GPU)
+    assert got.shape == (1001, 2560)
+    assert np.array_equal(masks, rmask)
+    assert np.array_equal(got, ref)
+    m, snr, t = R.best_dm(got)
+    assert abs(m - R.FRB_DM) <= 2.5 and abs(t - 801) <= 2
 
 
 def test_drop_in_call_finds_the_burst(gpu_lib):
