@@ -297,6 +297,7 @@ struct af_plan {
   // cleaning per super-tile: d_xT holds one super-tile's rows [k*T2,
   // k*T2 + Lp) (channel-major, row pitch Lp); tile_items[tile_off[k] ..
   // tile_off[k+1]) are the clean items clipped to those rows
+  bool seg_is_win = false;               // segment s is window s: segmask aliases the masks
   bool clean_tiled = false;
   std::vector<af::RowItem> tile_items;
   std::vector<int> tile_off;

@@ -445,6 +445,11 @@ static af_status build(af_plan *p) {
     p->segs.push_back(s);
   }
   p->nseg = (int)p->segs.size();
+  // segments that are the windows themselves (no overlapping hops): the
+  // per-segment mask is the window mask, with no kernel to form it
+  p->seg_is_win = p->nseg == p->nwin;
+  for (int s = 0; s < p->nseg && p->seg_is_win; ++s)
+    p->seg_is_win = p->segs[s].w0 == (uint32_t)s && p->segs[s].w1 == (uint32_t)s + 1;
   p->wseg.assign(2 * (size_t)p->nwin, 0);
   for (int w = 0; w < p->nwin; ++w) {
     int a = -1, b = -1;
@@ -564,8 +569,12 @@ static af_status build(af_plan *p) {
     p->d_flags = (uint8_t *)q;
     AF_TRY(dalloc(p, &q, (size_t)p->nwin * d.nchan));
     p->d_masks = (uint8_t *)q;
-    AF_TRY(dalloc(p, &q, (size_t)p->nseg * d.nchan));
-    p->d_segmask = (uint8_t *)q;
+    if (p->seg_is_win) {
+      p->d_segmask = p->d_masks;
+    } else {
+      AF_TRY(dalloc(p, &q, (size_t)p->nseg * d.nchan));
+      p->d_segmask = (uint8_t *)q;
+    }
   }
   AF_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
   return AF_OK;
@@ -695,7 +704,8 @@ af_status run_iqrm_vote(af_plan *p, hipStream_t st) {
   }
   AF_HIP(launch_iqrm_merge(p->d_flags, p->d_lags, p->nlags, p->nwin, p->Csub, d.iqrm_c0,
                            d.nchan, p->d_masks, p->lag_reach, st));
-  AF_HIP(launch_segmask(p->d_masks, p->d_segs, p->nseg, d.nchan, p->d_segmask, st));
+  if (!p->seg_is_win)
+    AF_HIP(launch_segmask(p->d_masks, p->d_segs, p->nseg, d.nchan, p->d_segmask, st));
   return AF_OK;
 }
 
