@@ -334,7 +334,8 @@ class Plan:
                                     C.byref(ms)), "af_time_execute")
         return ms.value
 
-    def set_profiling(self, on: bool):
+    def set_profiling(self, on):
+        """False/0 off, True/1 every stage, 2 the dedispersion stage only."""
         check(lib().af_plan_set_profiling(self._h, int(on)))
 
     def stage_times(self) -> dict:

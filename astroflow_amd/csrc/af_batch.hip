@@ -97,8 +97,8 @@ af_status af_batch_create(af_plan *const *plans, int nplans, const void *const *
       delete b;
       return fail(AF_ERR_INVALID, "all plans of a batch must be on the current device");
     }
-    const bool prof = p->profiling;
-    p->profiling = false;
+    const int prof = p->profiling;
+    p->profiling = 0;
     g_record = &recs[i];
     af_status s = execute(p, d_ins[i], d_outs[i], d_masks ? d_masks[i] : nullptr, nullptr);
     g_record = nullptr;

@@ -315,7 +315,7 @@ struct af_plan {
   size_t scratch_bytes = 0;
 
   hipStream_t stream = nullptr;
-  bool profiling = false;
+  int profiling = 0;            // 0 off, 1 every stage, 2 the dedispersion stage only
   // one set of kNumEv events per profiled execute, recorded between stages
   static constexpr int kNumEv = 7;
   std::vector<hipEvent_t> ev;   // pool, kNumEv per set

@@ -580,13 +580,29 @@ static af_status build(af_plan *p) {
   return AF_OK;
 }
 
+// Stage k of a profiled execute lies between events k and k+1; the
+// dedispersion stage (first-stage kernels) between events 4 and 5.  Mode 2
+// records only those two.
 static af_status record(af_plan *p, int set, int k, hipStream_t st) {
   if (!p->profiling || set < 0) return AF_OK;
+  if (p->profiling == 2 && k != 4 && k != 5) return AF_OK;
   AF_HIP(hipEventRecord(p->ev[(size_t)set * af_plan::kNumEv + k], st));
   return AF_OK;
 }
 
 static af_status collect(af_plan *p) {
+  if (p->profiling == 2) {
+    for (int s = 0; s < p->ev_used; ++s) {
+      hipEvent_t *e = &p->ev[(size_t)s * af_plan::kNumEv];
+      AF_HIP(hipEventSynchronize(e[5]));
+      float ms = 0;
+      AF_HIP(hipEventElapsedTime(&ms, e[4], e[5]));
+      p->acc_ms[4] += ms;
+      p->acc_n += 1;
+    }
+    p->ev_used = 0;
+    return AF_OK;
+  }
   for (int s = 0; s < p->ev_used; ++s) {
     hipEvent_t *e = &p->ev[(size_t)s * af_plan::kNumEv];
     AF_HIP(hipEventSynchronize(e[af_plan::kNumEv - 1]));
@@ -856,13 +872,17 @@ af_status af_plan_set_profiling(af_plan *p, int enable) {
   if (!p) return fail(AF_ERR_INVALID, "null plan");
   (void)hipSetDevice(p->device);
   if (p->ev_used) AF_TRY(collect(p));
-  p->profiling = enable != 0;
+  if (enable < 0 || enable > 2) return fail(AF_ERR_INVALID, "profiling mode must be 0, 1 or 2");
+  p->profiling = enable;
   std::fill(std::begin(p->acc_ms), std::end(p->acc_ms), 0.0);
   p->acc_total = 0;
   p->acc_n = 0;
   if (p->profiling && p->ev.empty()) {
     p->ev.resize((size_t)64 * af_plan::kNumEv);
-    for (auto &e : p->ev) AF_HIP(hipEventCreate(&e));
+    // timing-only events: no system-scope fence (cache write-back) when they
+    // complete, which otherwise idles the GPU for several microseconds at
+    // every stage boundary of a profiled execute
+    for (auto &e : p->ev) AF_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
   }
   return AF_OK;
 }

@@ -130,7 +130,10 @@ af_status af_plan_query(const af_plan *plan, af_plan_info *info);
 af_status af_plan_tables(const af_plan *plan, int32_t *delays, int32_t *residuals);
 /* IQRM window bounds [t0,t1) in binned samples, nwin entries each. */
 af_status af_plan_windows(const af_plan *plan, uint32_t *t0s, uint32_t *t1s);
-/* Switch stage timing on (and reset it) or off. */
+/* Stage timing with HIP events on the execute stream, reset on each call:
+ * enable 0 off, 1 every stage (seven events per execute), 2 the dedispersion
+ * stage only (two events per execute; the other stage times read 0).  Each
+ * recorded event idles the GPU for a few microseconds at its stage boundary. */
 af_status af_plan_set_profiling(af_plan *plan, int enable);
 /* Synchronises with the recorded events and returns the averages. */
 af_status af_plan_stage_times(af_plan *plan, af_stage_times *t);
