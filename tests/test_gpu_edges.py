@@ -112,3 +112,35 @@ def test_invalid_descriptions_are_rejected(gpu_lib):
             capi.Plan(spec)
     # a valid description still works afterwards
     capi.Plan(make_spec(x, b, 0.0, 1.0, 10, method="direct")).close()
+
+
+def test_profiling_modes(gpu_lib):
+    say("case: stage timing modes 1 (every stage) and 2 (the dedispersion stage only)")
+    rng = np.random.default_rng(44)
+    b = small_band(256, 1200.0, 1500.0, 1e-4)
+    x = noise(rng, 4000, 256)
+    spec = make_spec(x, b, 0.0, 1.0, 40, method="subband", sbw=32, ndm0=8,
+                     use_iqrm=True, iqrm_c0=0, iqrm_c1=255, use_zero_dm=True, zdm_c0=0, zdm_c1=256)
+    plan = capi.Plan(spec)
+    d_in = capi.DeviceBuffer(x.nbytes)
+    d_in.upload(x)
+    d_out = capi.DeviceBuffer(int(np.prod(plan.out_shape)) * 4)
+    plan.execute(d_in.ptr, d_out.ptr)
+    capi.sync()
+    ref = d_out.download(np.empty(plan.out_shape, np.uint32))
+    for mode in (1, 2):
+        plan.set_profiling(mode)
+        for _ in range(3):
+            plan.execute(d_in.ptr, d_out.ptr)
+        t = plan.stage_times()
+        assert t["n"] == 3 and t["dedisp"] > 0
+        if mode == 1:
+            assert t["stats"] > 0 and t["clean"] > 0 and t["stage2"] > 0 and t["total"] > 0
+        else:
+            assert t["stats"] == 0 and t["clean"] == 0 and t["stage2"] == 0
+        # timing does not change the results
+        assert np.array_equal(d_out.download(np.empty(plan.out_shape, np.uint32)), ref)
+    with pytest.raises(ValueError, match="profiling mode"):
+        plan.set_profiling(3)
+    plan.set_profiling(0)
+    plan.close()
