@@ -324,6 +324,12 @@ struct af_plan {
   double acc_total = 0;
   int acc_n = 0;
   std::vector<void *> allocs;
+  // af_execute_host: device buffers kept between calls (grow-only) and two
+  // page-locked staging buffers with their events
+  void *hx_in = nullptr, *hx_out = nullptr, *hx_masks = nullptr;
+  size_t hx_in_n = 0, hx_out_n = 0, hx_masks_n = 0;
+  void *stage[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
 
 namespace af {

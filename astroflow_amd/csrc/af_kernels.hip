@@ -260,17 +260,15 @@ __device__ __forceinline__ void lag_sort(const float *xs, float *ds_, float *srt
     }
     v[r] = dv;
   }
-#pragma unroll
-  for (int kk = 2; kk <= P2; kk <<= 1) {
-#pragma unroll
-    for (int j = kk >> 1; j > 0; j >>= 1) {
+  // one pass of the network: sequence length kk, distance j
+  auto pass = [&](const int kk, const int j) {
       if (j < E) {
         // compile-time distances keep v[] in registers
         if (j == 1) sort_pass_regs<E, 1>(v, t, kk);
         else if (j == 2) sort_pass_regs<E, 2>(v, t, kk);
         else if (j == 4) sort_pass_regs<E, 4>(v, t, kk);
         else sort_pass_regs<E, 8>(v, t, kk);
-        continue;
+        return;
       }
       const int m = j / E;   // partner thread t ^ m holds the other element
       float pv[E];
@@ -292,7 +290,15 @@ __device__ __forceinline__ void lag_sort(const float *xs, float *ds_, float *srt
         const float mn = fminf(v[r], pv[r]), mx = fmaxf(v[r], pv[r]);
         v[r] = (lower == asc) ? mn : mx;
       }
-    }
+  };
+  if constexpr (P2C > 0) {
+#pragma unroll
+    for (int kk = 2; kk <= P2C; kk <<= 1)
+#pragma unroll
+      for (int j = kk >> 1; j > 0; j >>= 1) pass(kk, j);
+  } else {
+    for (int kk = 2; kk <= P2; kk <<= 1)
+      for (int j = kk >> 1; j > 0; j >>= 1) pass(kk, j);
   }
   __syncthreads();
 #pragma unroll

@@ -8,6 +8,8 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <thread>
+#include <vector>
 
 #include "af_internal.h"
 
@@ -125,6 +127,14 @@ static af_status dalloc(af_plan *p, void **dst, size_t bytes) {
 static void release(af_plan *p) {
   for (void *q : p->allocs) (void)hipFree(q);
   p->allocs.clear();
+  for (void **q : {&p->hx_in, &p->hx_out, &p->hx_masks})
+    if (*q) (void)hipFree(*q), *q = nullptr;
+  for (int b = 0; b < 2; ++b) {
+    if (p->stage[b]) (void)hipHostFree(p->stage[b]);
+    if (p->stage_ev[b]) (void)hipEventDestroy(p->stage_ev[b]);
+    p->stage[b] = nullptr;
+    p->stage_ev[b] = nullptr;
+  }
   for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
   p->ev.clear();
   if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -926,38 +936,139 @@ af_status af_execute_spectrum(af_plan *p, const void *d_in, void *d_out, size_t 
   return AF_OK;
 }
 
+namespace {
+
+constexpr size_t kStageBytes = (size_t)32 << 20;   // per page-locked staging buffer
+
+// memcpy split over up to 8 host threads (pageable user memory <-> page-
+// locked staging: one thread does not reach the link's rate)
+void par_copy(void *dst, const void *src, size_t n) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t nt = std::min<size_t>({8, hw, std::max<size_t>(1, n >> 22)});
+  if (nt <= 1) {
+    std::memcpy(dst, src, n);
+    return;
+  }
+  const size_t piece = (n / nt + 4095) & ~(size_t)4095;
+  std::vector<std::thread> th;
+  th.reserve(nt - 1);
+  for (size_t t = 1; t < nt; ++t) {
+    const size_t a = t * piece;
+    if (a >= n) break;
+    const size_t m = std::min(piece, n - a);
+    th.emplace_back([=] { std::memcpy((char *)dst + a, (const char *)src + a, m); });
+  }
+  std::memcpy(dst, src, std::min(piece, n));
+  for (auto &x : th) x.join();
+}
+
+af_status grow(void **buf, size_t *have, size_t need) {
+  if (*have >= need && *buf) return AF_OK;
+  if (*buf) (void)hipFree(*buf);
+  *buf = nullptr;
+  *have = 0;
+  if (hipMalloc(buf, std::max<size_t>(need, 1)) != hipSuccess) {
+    *buf = nullptr;
+    return fail(AF_ERR_NOMEM, "hipMalloc of " + std::to_string(need) + " bytes for a host call");
+  }
+  *have = need;
+  return AF_OK;
+}
+
+// page-locked staging for af_execute_host; false when it cannot be had
+bool staging(af_plan *p) {
+  for (int b = 0; b < 2; ++b) {
+    if (!p->stage[b] && hipHostMalloc(&p->stage[b], kStageBytes, hipHostMallocDefault) != hipSuccess) {
+      p->stage[b] = nullptr;
+      return false;
+    }
+    if (!p->stage_ev[b] &&
+        hipEventCreateWithFlags(&p->stage_ev[b], hipEventDisableTiming) != hipSuccess) {
+      p->stage_ev[b] = nullptr;
+      return false;
+    }
+  }
+  return true;
+}
+
+// host -> device through the two staging buffers: the copy of chunk k into
+// its buffer overlaps the DMA of chunk k-1
+hipError_t upload_staged(af_plan *p, void *d, const void *h, size_t n, hipStream_t st) {
+  for (size_t off = 0, k = 0; off < n; off += kStageBytes, ++k) {
+    const size_t m = std::min(kStageBytes, n - off);
+    const int b = (int)(k & 1);
+    hipError_t e = hipEventSynchronize(p->stage_ev[b]);   // its previous DMA is done
+    if (e != hipSuccess) return e;
+    par_copy(p->stage[b], (const char *)h + off, m);
+    e = hipMemcpyAsync((char *)d + off, p->stage[b], m, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipEventRecord(p->stage_ev[b], st);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// device -> host: the DMA of chunk k overlaps the copy-out of chunk k-1
+hipError_t download_staged(af_plan *p, void *h, const void *d, size_t n, hipStream_t st) {
+  size_t prev_off = 0, prev_n = 0;
+  int prev_b = -1;
+  for (size_t off = 0, k = 0; off < n || prev_b >= 0; off += kStageBytes, ++k) {
+    const int b = (int)(k & 1);
+    const size_t m = off < n ? std::min(kStageBytes, n - off) : 0;
+    if (m) {
+      hipError_t e = hipMemcpyAsync(p->stage[b], (const char *)d + off, m, hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipEventRecord(p->stage_ev[b], st);
+      if (e != hipSuccess) return e;
+    }
+    if (prev_b >= 0) {
+      const hipError_t e = hipEventSynchronize(p->stage_ev[prev_b]);
+      if (e != hipSuccess) return e;
+      par_copy((char *)h + prev_off, p->stage[prev_b], prev_n);
+    }
+    prev_b = m ? b : -1;
+    prev_off = off;
+    prev_n = m;
+  }
+  return hipSuccess;
+}
+
+}  // namespace
+
 af_status af_execute_host(af_plan *p, const void *h_in, void *h_out, uint8_t *h_masks) {
   if (!p || !h_in || !h_out) return fail(AF_ERR_INVALID, "null argument");
   AF_HIP(hipSetDevice(p->device));
   const size_t in_bytes = p->d.nsamp * (size_t)p->d.nchan * dtype_size(p->d.dtype);
   const size_t out_bytes = (size_t)p->ndm * p->nout * acc_size(p->d.dtype);
   const size_t m_bytes = (size_t)p->nwin * p->d.nchan;
-  void *din = nullptr, *dout = nullptr, *dm = nullptr;
-  hipError_t e = hipMalloc(&din, in_bytes);
-  if (e != hipSuccess) return fail(AF_ERR_NOMEM, "hipMalloc input");
-  e = hipMalloc(&dout, out_bytes);
-  if (e != hipSuccess) { (void)hipFree(din); return fail(AF_ERR_NOMEM, "hipMalloc output"); }
+  // device buffers stay with the plan for the next call
+  AF_TRY(grow(&p->hx_in, &p->hx_in_n, in_bytes));
+  AF_TRY(grow(&p->hx_out, &p->hx_out_n, out_bytes));
+  uint8_t *dm = nullptr;
   if (h_masks && m_bytes) {
-    e = hipMalloc(&dm, m_bytes);
-    if (e != hipSuccess) { (void)hipFree(din); (void)hipFree(dout); return fail(AF_ERR_NOMEM, "hipMalloc masks"); }
+    AF_TRY(grow(&p->hx_masks, &p->hx_masks_n, m_bytes));
+    dm = (uint8_t *)p->hx_masks;
   }
-  af_status s = AF_OK;
   hipStream_t st = p->stream;
-  e = hipMemcpyAsync(din, h_in, in_bytes, hipMemcpyHostToDevice, st);
-  if (e != hipSuccess) s = fail(AF_ERR_RUNTIME, std::string("H2D: ") + hipGetErrorString(e));
-  if (s == AF_OK) s = execute(p, din, dout, (uint8_t *)dm, st);
-  if (s == AF_OK) {
-    e = hipMemcpyAsync(h_out, dout, out_bytes, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess && dm) e = hipMemcpyAsync(h_masks, dm, m_bytes, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) s = fail(AF_ERR_RUNTIME, std::string("D2H: ") + hipGetErrorString(e));
-  } else {
+  const bool staged = staging(p);
+  hipError_t e = staged ? upload_staged(p, p->hx_in, h_in, in_bytes, st)
+                        : hipMemcpyAsync(p->hx_in, h_in, in_bytes, hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) {
     (void)hipStreamSynchronize(st);
+    return fail(AF_ERR_RUNTIME, std::string("H2D: ") + hipGetErrorString(e));
   }
-  (void)hipFree(din);
-  (void)hipFree(dout);
-  if (dm) (void)hipFree(dm);
-  return s;
+  af_status s = execute(p, p->hx_in, p->hx_out, dm, st);
+  if (s != AF_OK) {
+    (void)hipStreamSynchronize(st);
+    return s;
+  }
+  e = staged ? download_staged(p, h_out, p->hx_out, out_bytes, st)
+             : hipMemcpyAsync(h_out, p->hx_out, out_bytes, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess && dm) e = hipMemcpyAsync(h_masks, dm, m_bytes, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    (void)hipStreamSynchronize(st);
+    return fail(AF_ERR_RUNTIME, std::string("D2H: ") + hipGetErrorString(e));
+  }
+  return AF_OK;
 }
 
 af_status af_time_execute(af_plan *p, const void *d_in, void *d_out, int reps, float *ms) {
