@@ -16,6 +16,7 @@
 #include <cstring>
 #include <fstream>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -226,6 +227,79 @@ struct Raw {
   Prepared prep;
 };
 
+// Plans of recent calls, keyed by their full description (scalar fields,
+// frequency table, static mask, device), most recent last.  A call takes its
+// plan out while it runs (plans are confined to one thread) and puts it back
+// afterwards; the oldest beyond kPlanCacheMax are destroyed.  Astroflow's
+// search loop calls with the same parameters chunk after chunk: the plan,
+// its tables and its host-call buffers are then built once.
+constexpr size_t kPlanCacheMax = 2;
+std::mutex g_plan_mu;
+std::vector<std::pair<std::string, af_plan *>> g_plans;
+
+std::string plan_key(const Prepared &pr, int dev) {
+  af_plan_desc d = pr.d;
+  d.freq_mhz = nullptr;
+  d.bad_chans = nullptr;
+  std::string k((const char *)&d, sizeof d);
+  k.append((const char *)pr.freq.data(), pr.freq.size() * sizeof(double));
+  k.append((const char *)pr.bad.data(), pr.bad.size() * sizeof(int));
+  k.append((const char *)&dev, sizeof dev);
+  return k;
+}
+
+af_plan *take_plan(const std::string &key) {
+  std::lock_guard<std::mutex> g(g_plan_mu);
+  for (size_t i = g_plans.size(); i-- > 0;)
+    if (g_plans[i].first == key) {
+      af_plan *p = g_plans[i].second;
+      g_plans.erase(g_plans.begin() + (std::ptrdiff_t)i);
+      return p;
+    }
+  return nullptr;
+}
+
+void give_back(std::string key, af_plan *p) {
+  std::vector<af_plan *> drop;
+  {
+    std::lock_guard<std::mutex> g(g_plan_mu);
+    g_plans.emplace_back(std::move(key), p);
+    while (g_plans.size() > kPlanCacheMax) {
+      drop.push_back(g_plans.front().second);
+      g_plans.erase(g_plans.begin());
+    }
+  }
+  for (af_plan *q : drop) af_plan_destroy(q);
+}
+
+void clear_plan_cache() {
+  std::vector<std::pair<std::string, af_plan *>> all;
+  {
+    std::lock_guard<std::mutex> g(g_plan_mu);
+    all.swap(g_plans);
+  }
+  for (auto &e : all) af_plan_destroy(e.second);
+}
+
+// A cached or new plan for one call: back to the cache if the call
+// succeeded, destroyed otherwise.
+struct PlanLease {
+  std::string key;
+  af_plan *plan = nullptr;
+  bool ok = false;
+  PlanLease(const Prepared &pr, int dev) : key(plan_key(pr, dev)) {
+    plan = take_plan(key);
+    if (!plan) check(af_plan_create(&pr.d, &plan), "af_plan_create");
+  }
+  ~PlanLease() {
+    if (!plan) return;
+    if (ok) give_back(std::move(key), plan);
+    else af_plan_destroy(plan);
+  }
+  PlanLease(const PlanLease &) = delete;
+  PlanLease &operator=(const PlanLease &) = delete;
+};
+
 template <typename T>
 Raw run(py::array_t<T, py::array::c_style> data, const Header &h, float dm_low, float dm_high,
         float freq_start, float freq_end, float dm_step, int ds, int target_id,
@@ -242,9 +316,8 @@ Raw run(py::array_t<T, py::array::c_style> data, const Header &h, float dm_low, 
   Raw r;
   r.prep = prepare(dtype_code<T>(), h, (size_t)h.ndata, dm_low, dm_high, freq_start, freq_end,
                    dm_step, ds, mask_file, rfi);
-  af_plan *plan = nullptr;
-  check(af_plan_create(&r.prep.d, &plan), "af_plan_create");
-  std::unique_ptr<af_plan, decltype(&af_plan_destroy)> guard(plan, af_plan_destroy);
+  PlanLease lease(r.prep, target_id);
+  af_plan *plan = lease.plan;
   af_plan_info info;
   check(af_plan_query(plan, &info), "af_plan_query");
   using O = typename OutOf<T>::type;
@@ -264,6 +337,7 @@ Raw run(py::array_t<T, py::array::c_style> data, const Header &h, float dm_low, 
     st = af_execute_host(plan, src, dst, mp);
   }
   check(st, "af_execute_host");
+  lease.ok = true;
   r.plane = plane;
   r.masks = masks;
   return r;
@@ -696,6 +770,14 @@ py::tuple calculate_frb_snr(py::array spec, py::object noise_range, double thres
 
 PYBIND11_MODULE(_astroflow_core, m) {
   m.doc() = "astroflow dedispersion core on MI355X (gfx950), over libastroflow_amd";
+
+  m.def("_plan_cache_size", [] {
+    std::lock_guard<std::mutex> g(g_plan_mu);
+    return g_plans.size();
+  }, "plans kept from recent _dedisperse_spec / _dedispered_fil calls");
+  m.def("_plan_cache_clear", &clear_plan_cache, "destroy the kept plans (frees their device memory)");
+  // destroy kept plans before interpreter shutdown, while the HIP runtime is up
+  py::module_::import("atexit").attr("register")(py::cpp_function(&clear_plan_cache));
 
   def_result<DedispersedDataUint8>(m, "DedispersedDataUint8");
   def_result<DedisperedData>(m, "DedisperedData");

@@ -188,3 +188,33 @@ def test_spectrum_at_dm_zero_is_the_input(gpu_lib):
                                 core.RFIconfig(), [])
     assert sp.data.shape == (NSAMP, cx - cs)
     assert np.array_equal(sp.data, x[:, cs:cx])
+
+
+def test_repeated_calls_reuse_the_plan(gpu_lib):
+    say("case: plan cache of the module: repeated and new parameter sets")
+    x = data_u8()
+    q = core.IQRMconfig(1, 0.03, 3.0, 1.5, 0.12, 0.06, True)
+    rfi = core.RFIconfig(False, True, True, q)
+    core._plan_cache_clear()
+    assert core._plan_cache_size() == 0
+    args = (0.0, 120.0, 1210.0, 1490.0, 1.0, 2)
+    a, ma = core._dedisperse_spec_raw(x, header(), *args, 0, "", rfi, True)
+    assert core._plan_cache_size() == 1
+    b, mb = core._dedisperse_spec_raw(x, header(), *args, 0, "", rfi, True)
+    assert core._plan_cache_size() == 1
+    assert np.array_equal(a, b) and np.array_equal(ma, mb)
+    ref, rmask, *_ = oracle_for_call(x, *args, rfi, [])
+    assert np.array_equal(b, ref) and np.array_equal(mb, rmask)
+    # other parameters: another plan, the oldest dropped beyond two
+    c = core._dedisperse_spec_raw(x, header(), 5.0, 60.0, 1210.0, 1490.0, 1.0, 1, 0, "", rfi)
+    assert core._plan_cache_size() == 2
+    core._dedisperse_spec_raw(x, header(), 0.0, 30.0, 1210.0, 1490.0, 1.0, 1, 0, "", rfi)
+    assert core._plan_cache_size() == 2
+    ref_c, *_ = oracle_for_call(x, 5.0, 60.0, 1210.0, 1490.0, 1.0, 1, rfi, [])
+    assert np.array_equal(c, ref_c)
+    # a failing call keeps no plan
+    with pytest.raises(ValueError):
+        core._dedisperse_spec_raw(x, header(), 0.0, 30.0, 900.0, 1000.0, 1.0, 1, 0, "", rfi)
+    assert core._plan_cache_size() == 2
+    core._plan_cache_clear()
+    assert core._plan_cache_size() == 0
