@@ -475,8 +475,9 @@ def log(msg):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not run)"}))
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1 (the path "
+                                   "shards by beam or DM range with no exchange; that layout "
+                                   "is not run here)"}))
         sys.exit(2)
     from astroflow_amd import capi
 
