@@ -262,34 +262,34 @@ __device__ __forceinline__ void lag_sort(const float *xs, float *ds_, float *srt
   }
   // one pass of the network: sequence length kk, distance j
   auto pass = [&](const int kk, const int j) {
-      if (j < E) {
-        // compile-time distances keep v[] in registers
-        if (j == 1) sort_pass_regs<E, 1>(v, t, kk);
-        else if (j == 2) sort_pass_regs<E, 2>(v, t, kk);
-        else if (j == 4) sort_pass_regs<E, 4>(v, t, kk);
-        else sort_pass_regs<E, 8>(v, t, kk);
-        return;
-      }
-      const int m = j / E;   // partner thread t ^ m holds the other element
-      float pv[E];
-      if (m < 64) {
+    if (j < E) {
+      // compile-time distances keep v[] in registers
+      if (j == 1) sort_pass_regs<E, 1>(v, t, kk);
+      else if (j == 2) sort_pass_regs<E, 2>(v, t, kk);
+      else if (j == 4) sort_pass_regs<E, 4>(v, t, kk);
+      else sort_pass_regs<E, 8>(v, t, kk);
+      return;
+    }
+    const int m = j / E;   // partner thread t ^ m holds the other element
+    float pv[E];
+    if (m < 64) {
 #pragma unroll
-        for (int r = 0; r < E; ++r) pv[r] = __shfl_xor(v[r], m);
-      } else {
-        __syncthreads();
+      for (int r = 0; r < E; ++r) pv[r] = __shfl_xor(v[r], m);
+    } else {
+      __syncthreads();
 #pragma unroll
-        for (int r = 0; r < E; ++r) srt[E * t + r] = v[r];
-        __syncthreads();
+      for (int r = 0; r < E; ++r) srt[E * t + r] = v[r];
+      __syncthreads();
 #pragma unroll
-        for (int r = 0; r < E; ++r) pv[r] = srt[E * (t ^ m) + r];
-      }
-      const bool lower = (t & m) == 0;
+      for (int r = 0; r < E; ++r) pv[r] = srt[E * (t ^ m) + r];
+    }
+    const bool lower = (t & m) == 0;
 #pragma unroll
-      for (int r = 0; r < E; ++r) {
-        const bool asc = ((E * (lower ? t : (t ^ m)) + r) & kk) == 0;
-        const float mn = fminf(v[r], pv[r]), mx = fmaxf(v[r], pv[r]);
-        v[r] = (lower == asc) ? mn : mx;
-      }
+    for (int r = 0; r < E; ++r) {
+      const bool asc = ((E * (lower ? t : (t ^ m)) + r) & kk) == 0;
+      const float mn = fminf(v[r], pv[r]), mx = fmaxf(v[r], pv[r]);
+      v[r] = (lower == asc) ? mn : mx;
+    }
   };
   if constexpr (P2C > 0) {
 #pragma unroll
