@@ -156,7 +156,10 @@ af_status af_execute(af_plan *plan, const void *d_in, void *d_out,
 af_status af_execute_spectrum(af_plan *plan, const void *d_in, void *d_out, size_t t0,
                               size_t t_len, uint8_t *d_masks, void *stream);
 
-/* Same on host buffers: upload, execute, download, synchronise. */
+/* Same on host buffers: upload, execute, download, synchronise.  The plan
+ * keeps device buffers for the input and the plane from its first host call
+ * on (grown when needed) and moves the data through two 32 MB page-locked
+ * staging buffers; all are freed by af_plan_destroy. */
 af_status af_execute_host(af_plan *plan, const void *h_in, void *h_out,
                           uint8_t *h_masks);
 
