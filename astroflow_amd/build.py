@@ -42,13 +42,27 @@ def _run(cmd, cwd=None):
     return out
 
 
+OBJ_DIR = os.path.join(PKG, "_build")
+
+
 def build_lib(verbose=False):
+    """Compiles every .hip source to an object (in parallel: one hipcc per
+    source), then links the shared library."""
+    from concurrent.futures import ThreadPoolExecutor
     srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
     extra = [os.path.join(CSRC, s) for s in os.listdir(CSRC)
              if s.endswith(".hip") and s not in HIP_SOURCES]
+    srcs += sorted(extra)
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    objs = [os.path.join(OBJ_DIR, os.path.basename(s)[:-4] + ".o") for s in srcs]
+    jobs = max(1, min(len(srcs), int(os.environ.get("MAX_JOBS", "0")) or (os.cpu_count() or 1)))
+    with ThreadPoolExecutor(jobs) as ex:
+        outs = list(ex.map(lambda so: _run([HIPCC, *HIP_FLAGS, "-c", so[0], "-o", so[1]]),
+                           zip(srcs, objs)))
     tmp = LIB + ".tmp"
-    out = _run([HIPCC, *HIP_FLAGS, "-shared", *srcs, *sorted(extra), "-o", tmp])
+    outs.append(_run([HIPCC, f"--offload-arch={ARCH}", "-shared", *objs, "-o", tmp]))
     os.replace(tmp, LIB)
+    out = "\n".join(o for o in outs if o.strip())
     if verbose and out.strip():
         print(out)
     return LIB

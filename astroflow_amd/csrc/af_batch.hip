@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -86,6 +87,16 @@ af_status af_batch_create(af_plan *const *plans, int nplans, const void *const *
   *batch = nullptr;
   for (int i = 0; i < nplans; ++i)
     if (!plans[i] || !d_ins[i] || !d_outs[i]) return fail(AF_ERR_INVALID, "null plan or buffer");
+  // each entry runs on its plan's own scratch (cleaned buffer, intermediate,
+  // IQRM partials and masks): a plan given twice would have both entries'
+  // merged launches write the same scratch at once
+  {
+    std::vector<af_plan *> seen(plans, plans + nplans);
+    std::sort(seen.begin(), seen.end());
+    if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+      return fail(AF_ERR_INVALID, "a plan appears more than once in the batch (one plan per "
+                                  "entry: same-shape chunks need a plan each)");
+  }
   af_batch *b = new af_batch();
   b->plans.assign(plans, plans + nplans);
   (void)hipGetDevice(&b->device);
@@ -275,25 +286,37 @@ af_status af_batch_destroy(af_batch *b) {
 namespace af {
 namespace {
 
-struct TuneKey {
-  int dtype, nchan, ds, dd_c0, dd_c1, ndm, method, sb_nchan, use_iqrm, tune_sb;
-  size_t nsamp, nout;
-  double dm_step;
-  bool operator<(const TuneKey &o) const {
-    return std::tie(dtype, nchan, ds, dd_c0, dd_c1, ndm, method, sb_nchan, use_iqrm, tune_sb, nsamp,
-                    nout, dm_step) < std::tie(o.dtype, o.nchan, o.ds, o.dd_c0, o.dd_c1, o.ndm,
-                                              o.method, o.sb_nchan, o.use_iqrm, o.tune_sb,
-                                              o.nsamp, o.nout, o.dm_step);
-  }
-};
+// Everything that shapes a plan and its timing except the settings being
+// tuned: the scalar fields one by one (no struct padding), the frequency
+// table, the static mask and the device.  tile_t and dm_per_block are
+// outputs; sb_nchan and ndm0 too when the subband layout is tuned.
+std::string tune_key(const af_plan_desc &d, int tune_sb, int dev) {
+  std::string k;
+  auto put = [&](const auto &v) { k.append((const char *)&v, sizeof v); };
+  put(d.dtype); put(d.nchan); put(d.nsamp); put(d.ds); put(d.tsamp); put(d.f_ref_mhz);
+  put(d.dd_c0); put(d.dd_c1); put(d.dm_low); put(d.dm_step); put(d.ndm); put(d.nout);
+  put(d.method);
+  const int sbw = tune_sb ? 0 : d.sb_nchan, nd0 = tune_sb ? 0 : d.ndm0;
+  put(sbw); put(nd0);
+  put(d.use_iqrm); put(d.iqrm_c0); put(d.iqrm_c1); put(d.iqrm_mode); put(d.iqrm_radius_frac);
+  put(d.iqrm_nsigma); put(d.iqrm_geofactor); put(d.iqrm_win); put(d.iqrm_hop);
+  put(d.iqrm_include_tail);
+  put(d.use_zero_dm); put(d.zdm_c0); put(d.zdm_c1);
+  put(d.inter_budget); put(d.clean_budget);
+  put(tune_sb); put(dev);
+  if (d.freq_mhz) k.append((const char *)d.freq_mhz, (size_t)std::max(0, d.nchan) * sizeof(double));
+  put(d.n_bad);
+  if (d.bad_chans && d.n_bad > 0) k.append((const char *)d.bad_chans, (size_t)d.n_bad * sizeof(int));
+  return k;
+}
 
 struct TuneVal {
-  int tile_t, dm_per_block, sb_nchan;
+  int tile_t, dm_per_block, sb_nchan, ndm0;
   float ms;
 };
 
 std::mutex g_tune_mu;
-std::map<TuneKey, TuneVal> g_tune;
+std::map<std::string, TuneVal> g_tune;
 
 }  // namespace
 }  // namespace af
@@ -302,9 +325,9 @@ extern "C" af_status af_autotune(af_plan_desc *desc, const void *d_in, void *d_o
                                  int tune_subbands, int reps, float *best_ms) {
   if (!desc || !d_in || !d_out) return fail(AF_ERR_INVALID, "null argument");
   if (reps < 1) reps = 1;
-  const TuneKey key{desc->dtype, desc->nchan, desc->ds, desc->dd_c0, desc->dd_c1, desc->ndm,
-                    desc->method, tune_subbands ? 0 : desc->sb_nchan, desc->use_iqrm,
-                    tune_subbands, desc->nsamp, desc->nout, desc->dm_step};
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const std::string key = tune_key(*desc, tune_subbands, dev);
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune.find(key);
@@ -312,40 +335,59 @@ extern "C" af_status af_autotune(af_plan_desc *desc, const void *d_in, void *d_o
       desc->tile_t = it->second.tile_t;
       desc->dm_per_block = it->second.dm_per_block;
       desc->sb_nchan = it->second.sb_nchan;
+      desc->ndm0 = it->second.ndm0;
       if (best_ms) *best_ms = it->second.ms;
       return AF_OK;
     }
   }
   const bool u8 = desc->dtype == AF_U8;
+  const bool f32 = desc->dtype == AF_F32;
   const bool sub = desc->method == AF_METHOD_SUBBAND;
+  // stage-1 form (u8): 0 = DM rows walking their windows through the L1,
+  // 16 = 16 rows sharing an LDS window
   std::vector<int> dpbs = {0};
   if (u8) dpbs.push_back(16);
+  // stage-2 output tile (0 = chosen by the plan from its LDS size)
   std::vector<int> tiles = {0};
   if (u8 && sub) tiles = {0, 256, 512, 1024};
-  std::vector<int> sbs = {desc->sb_nchan};
-  if (sub && tune_subbands) sbs = {16, 32, 64, 128};
-  TuneVal best{desc->tile_t, desc->dm_per_block, desc->sb_nchan, 1e30f};
+  if (f32 && sub) tiles = {0, 512, 1024};
+  // subband layout: channels per subband and fine DMs per coarse DM (these
+  // change the method's delay rounding, hence the plane)
+  std::vector<int> sbs = {desc->sb_nchan}, nd0s = {desc->ndm0};
+  if (sub && tune_subbands) {
+    sbs = {16, 32, 64, 128};
+    nd0s = {8, 16, 32, 64};
+  }
+  TuneVal best{desc->tile_t, desc->dm_per_block, desc->sb_nchan, desc->ndm0, 1e30f};
   for (int sb : sbs)
-    for (int dpb : dpbs)
-      for (int tile : tiles) {
-        af_plan_desc d = *desc;
-        d.sb_nchan = sb;
-        d.dm_per_block = dpb;
-        d.tile_t = tile;
-        af_plan *p = nullptr;
-        if (af_plan_create(&d, &p) != AF_OK) continue;   // a shape the kernels cannot take
-        float ms = 0;
-        af_status s = af_time_execute(p, d_in, d_out, 1, &ms);   // warm-up
-        if (s == AF_OK) s = af_time_execute(p, d_in, d_out, reps, &ms);
-        af_plan_destroy(p);
-        if (s != AF_OK) continue;
-        ms /= (float)reps;
-        if (ms < best.ms) best = TuneVal{tile, dpb, sb, ms};
-      }
+    for (int nd0 : nd0s)
+      for (int dpb : dpbs)
+        for (int tile : tiles) {
+          af_plan_desc d = *desc;
+          d.sb_nchan = sb;
+          d.ndm0 = nd0;
+          d.dm_per_block = dpb;
+          d.tile_t = tile;
+          af_plan *p = nullptr;
+          if (af_plan_create(&d, &p) != AF_OK) continue;   // a shape the kernels cannot take
+          // a requested tile the plan could not honour duplicates another candidate
+          if (tile != 0 && p->s2_tile != tile) {
+            af_plan_destroy(p);
+            continue;
+          }
+          float ms = 0;
+          af_status s = af_time_execute(p, d_in, d_out, 1, &ms);   // warm-up
+          if (s == AF_OK) s = af_time_execute(p, d_in, d_out, reps, &ms);
+          af_plan_destroy(p);
+          if (s != AF_OK) continue;
+          ms /= (float)reps;
+          if (ms < best.ms) best = TuneVal{tile, dpb, sb, nd0, ms};
+        }
   if (best.ms >= 1e29f) return fail(AF_ERR_INVALID, "no candidate plan could be built");
   desc->tile_t = best.tile_t;
   desc->dm_per_block = best.dm_per_block;
   desc->sb_nchan = best.sb_nchan;
+  desc->ndm0 = best.ndm0;
   if (best_ms) *best_ms = best.ms;
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tune[key] = best;

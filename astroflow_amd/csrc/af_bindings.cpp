@@ -139,6 +139,15 @@ struct Prepared {
   std::vector<double> freq;
   std::vector<int> bad;
   size_t chan_start = 0, chan_end = 0, dm_steps = 0, down_ndata = 0;
+  // d with its table pointers on this object's vectors (a copy of a
+  // Prepared keeps its own vectors)
+  af_plan_desc desc() const {
+    af_plan_desc x = d;
+    x.freq_mhz = freq.data();
+    x.bad_chans = bad.empty() ? nullptr : bad.data();
+    x.n_bad = (int)bad.size();
+    return x;
+  }
 };
 
 // Channel range, DM steps, reference frequency and RFI set-up exactly as
@@ -195,8 +204,10 @@ Prepared prepare(int dtype, const Header &h, size_t nsamp, float dm_low, float d
   d.method = AF_METHOD_SUBBAND;
   d.sb_nchan = 32;   // src/core/dedispered.cu:29-35
   d.ndm0 = 32;
-  d.use_iqrm = rfi.use_iqrm;
-  d.iqrm_c0 = (int)cs;   // IQRM takes the inclusive end as exclusive (:1089)
+  // IQRM takes the inclusive end as exclusive (:1089); an empty range gives
+  // no window masks in the reference (src/core/iqrm.cu:363-364)
+  d.use_iqrm = rfi.use_iqrm && ce > cs;
+  d.iqrm_c0 = (int)cs;
   d.iqrm_c1 = (int)ce;
   d.iqrm_mode = rfi.iqrm_cfg.mode;
   d.iqrm_radius_frac = rfi.iqrm_cfg.radius_frac;
@@ -289,7 +300,10 @@ struct PlanLease {
   bool ok = false;
   PlanLease(const Prepared &pr, int dev) : key(plan_key(pr, dev)) {
     plan = take_plan(key);
-    if (!plan) check(af_plan_create(&pr.d, &plan), "af_plan_create");
+    if (!plan) {
+      const af_plan_desc d = pr.desc();
+      check(af_plan_create(&d, &plan), "af_plan_create");
+    }
   }
   ~PlanLease() {
     if (!plan) return;
@@ -499,8 +513,10 @@ SpectrumOut spectrum_at_dm(py::array_t<T, py::array::c_style> data, Header h, fl
   std::vector<int> bad;
   if (rfi.use_mask) bad = read_mask_file(maskfile);
   d.nsamp = slen;
-  d.use_iqrm = rfi.use_iqrm;
-  d.iqrm_c0 = (int)cs;   // this path passes the exclusive end (include/cpucal.hpp:425-430)
+  // this path passes the exclusive end (include/cpucal.hpp:425-430); an
+  // empty range gives no window masks (src/core/iqrm.cu:363-364)
+  d.use_iqrm = rfi.use_iqrm && cx > cs;
+  d.iqrm_c0 = (int)cs;
   d.iqrm_c1 = (int)cx;
   d.iqrm_mode = rfi.iqrm_cfg.mode;
   d.iqrm_radius_frac = rfi.iqrm_cfg.radius_frac;
@@ -874,6 +890,37 @@ PYBIND11_MODULE(_astroflow_core, m) {
         py::arg("noise_range") = py::none(), py::arg("threshold_sigma") = 5.0,
         py::arg("toa_sample_idx") = py::none(), py::arg("fitting_window_samples") = py::none(),
         py::arg("tsamp") = py::none(), py::arg("target_time_us") = 256.0);
+
+  m.def("_describe_spec", [](size_t nsamp, const Header &h, float dm_low, float dm_high,
+                             float freq_start, float freq_end, float dm_step, int ds,
+                             const std::string &mask_file, const RFIConfig &rfi) {
+    // the plan description _dedisperse_spec derives from its arguments (no
+    // device needed; for tests of the parameter derivation)
+    const Prepared pr = prepare(AF_U8, h, nsamp, dm_low, dm_high, freq_start, freq_end, dm_step,
+                                ds, mask_file, rfi);
+    const af_plan_desc &d = pr.d;
+    py::dict o;
+    o["chan_start"] = pr.chan_start;
+    o["chan_end"] = pr.chan_end;
+    o["dm_steps"] = pr.dm_steps;
+    o["down_ndata"] = pr.down_ndata;
+    o["f_ref_mhz"] = d.f_ref_mhz;
+    o["dd_c0"] = d.dd_c0;
+    o["dd_c1"] = d.dd_c1;
+    o["use_iqrm"] = d.use_iqrm;
+    o["iqrm_c0"] = d.iqrm_c0;
+    o["iqrm_c1"] = d.iqrm_c1;
+    o["iqrm_win"] = d.iqrm_win;
+    o["iqrm_hop"] = d.iqrm_hop;
+    o["use_zero_dm"] = d.use_zero_dm;
+    o["zdm_c0"] = d.zdm_c0;
+    o["zdm_c1"] = d.zdm_c1;
+    o["bad_chans"] = pr.bad;
+    return o;
+  }, py::arg("nsamp"), py::arg("header"), py::arg("dm_low"), py::arg("dm_high"),
+     py::arg("freq_start"), py::arg("freq_end"), py::arg("dm_step") = 1.0f,
+     py::arg("time_downsample") = 1, py::arg("mask_file") = "mask.txt",
+     py::arg("rficfg") = RFIConfig());
 
   m.def("_image_slice", [](py::array_t<float, py::array::c_style> plane, int size) {
     // one DM-time image from a float plane (exposed for tests of the imaging)

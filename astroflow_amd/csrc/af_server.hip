@@ -110,6 +110,7 @@ struct af_server {
   // result planes: every allocation counts against the budget until destroy
   std::map<void *, size_t> plane_size;                // all planes
   std::map<size_t, std::vector<void *>> plane_free;   // free planes by size
+  std::map<void *, int> plane_held;                   // planes returned by poll, not yet released
   std::vector<hipEvent_t> ev_free;
 
   uint64_t submitted = 0, admitted = 0, completed = 0, planes = 0;
@@ -204,8 +205,9 @@ af_status fast_window(af_server *s, Beam &b, Flight &f, uint64_t k) {
   return AF_OK;
 }
 
-af_status dispatch(af_server *s, Flight &f) {
-  Beam &b = s->beams[f.r.beam];
+// b: the flight's beam, looked up under the lock (beams is appended to by
+// af_server_add_beam on other threads; references to its elements stay valid).
+af_status dispatch(af_server *s, Beam &b, Flight &f) {
   const size_t in_buf = (size_t)(b.total - b.head);
   if (in_buf + b.C > b.rcap) {
     // keep rows from the next window start; source and destination are
@@ -260,6 +262,7 @@ void finish(af_server *s, Flight &f, double t) {  // under lock
     s->lat.push_back(t - f.r.t_submit);
   }
   s->last_done = std::max(s->last_done, t);
+  if (f.plane) s->plane_held[f.plane] += 1;
   s->done.push_back(r);
   if (f.ev) s->ev_free.push_back(f.ev);
 }
@@ -308,7 +311,7 @@ void run(af_server *s) {
       f.t_admit = now_ms(s->t0);
       s->admitted++;
       lk.unlock();
-      af_status st = dispatch(s, f);
+      af_status st = dispatch(s, b, f);
       lk.lock();
       if (st != AF_OK) {
         f.status = st;
@@ -502,6 +505,8 @@ af_status af_server_add_beam(af_server *s, const af_plan_desc *w, size_t chunk_r
     st = fail(AF_ERR_RUNTIME, "hipStreamCreate");
   if (st == AF_OK && s->incremental) st = setup_fast(b, &extra);
   std::lock_guard<std::mutex> lk(s->mu);
+  // chunks submitted since the first check: the beam is not added
+  if (st == AF_OK && s->completed != s->submitted) st = fail(AF_ERR_BUSY, "chunks in flight");
   if (st == AF_OK && s->used + b.resident + extra > s->budget)
     st = fail(AF_ERR_NOMEM, "beam does not fit the HBM budget");
   if (st != AF_OK) {
@@ -548,6 +553,11 @@ af_status af_server_release(af_server *s, void *d_plane) {
   std::lock_guard<std::mutex> lk(s->mu);
   auto it = s->plane_size.find(d_plane);
   if (it == s->plane_size.end()) return fail(AF_ERR_INVALID, "not a result plane of this server");
+  auto h = s->plane_held.find(d_plane);
+  if (h == s->plane_held.end())
+    return fail(AF_ERR_INVALID, "plane is not held by the caller (already released, or its "
+                                "chunk has not been returned by af_server_poll)");
+  if (--h->second == 0) s->plane_held.erase(h);
   s->plane_free[it->second].push_back(d_plane);
   s->cv_work.notify_one();
   return AF_OK;

@@ -204,3 +204,63 @@ def c3_chunks(seed: int = 4, n: int = 48):
             dm_step=float(rng.choice([1.0, 0.5])),
             ds=int(rng.choice([1, 2, 4]))))
     return out
+
+
+def noise_f32_threaded(seed: int, nsamp: int, nchan: int, threads: int = 8,
+                       rows_per_block: int = 8192) -> np.ndarray:
+    """N(0,1) float32, generated in fixed blocks of rows from spawned seeds
+    (the result does not depend on the thread count)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    out = np.empty((nsamp, nchan), np.float32)
+    nblk = (nsamp + rows_per_block - 1) // rows_per_block
+    seeds = np.random.SeedSequence(seed).spawn(nblk)
+
+    def work(b):
+        r0 = b * rows_per_block
+        r1 = min(nsamp, r0 + rows_per_block)
+        np.random.default_rng(seeds[b]).standard_normal((r1 - r0, nchan), dtype=np.float32,
+                                                         out=out[r0:r1])
+
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        list(ex.map(work, range(nblk)))
+    return out
+
+
+def c3_chunk_data(c: dict, seed: int, threads: int = 8) -> np.ndarray:
+    """Seeded input of one batched beam-chunk (a dict from c3_chunks): uint8
+    N(128,16) or float32 N(0,1) noise, plus narrowband RFI on 5% of the
+    channels (sinusoids of amplitude 40 for uint8 and 2.5 sigma for float32,
+    periods log-uniform in 8-4096 samples) and 3 channels stuck at the
+    maximum (255 / 8.0), so that IQRM has channels to flag."""
+    nsamp, nchan = c["nsamp"], c["nchan"]
+    if c["dtype"] == np.uint8:
+        x = noise_u8_threaded(seed, nsamp, nchan, threads=threads)
+        amp, top = 40.0, 255.0
+    else:
+        x = noise_f32_threaded(seed, nsamp, nchan, threads=threads)
+        amp, top = 2.5, 8.0
+    rng = np.random.default_rng(seed + 7919)
+    k = max(1, int(round(0.05 * nchan)))
+    chans = rng.choice(nchan, size=k + 3, replace=False)
+    periods = np.exp(rng.uniform(math.log(8.0), math.log(4096.0), size=k))
+    phases = rng.uniform(0, 2 * math.pi, size=k)
+    from concurrent.futures import ThreadPoolExecutor
+    blk = 16384
+    sel = chans[:k]
+
+    def work(b):
+        r0 = b * blk
+        r1 = min(nsamp, r0 + blk)
+        t = np.arange(r0, r1, dtype=np.float64)[:, None]
+        v = x[r0:r1, sel].astype(np.float64)
+        v += amp * np.sin(2 * math.pi * t / periods[None, :] + phases[None, :])
+        if x.dtype == np.uint8:
+            x[r0:r1, sel] = np.clip(np.round(v), 0, 255).astype(np.uint8)
+        else:
+            x[r0:r1, sel] = v.astype(np.float32)
+        x[r0:r1, chans[k:]] = top
+
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        list(ex.map(work, range((nsamp + blk - 1) // blk)))
+    return x

@@ -32,8 +32,9 @@ UNIT = "DM-trial*samples/s"
 HBM_PEAK_GBS = 8000.0
 VALU_ADD_PEAK = 256 * 64 * 2.4e9     # 32-bit adds/s (one per lane per clock)
 STAGE1_KERNEL = "k_dd_u8_q"
-# FETCH_SIZE/WRITE_SIZE summary of this workload, scaled by the copy-kernel
-# calibration; one "traffic_bytes <kernel> <bytes>" line per kernel.
+# HBM bytes per dispatch from the L2-to-memory request counters of this
+# workload, calibrated on a copy of known size; one
+# "traffic_bytes <kernel> <bytes>" line per kernel.
 TRAFFIC_FILE = "profiles/hbm_traffic_4096ch_u8_subband_iqrm_zdm.md"
 
 
@@ -67,6 +68,9 @@ def parse():
     ap.add_argument("--no-direct", action="store_true",
                     help="skip the direct-method leg (1024 ch x 2^18, DM 0-1000)")
     ap.add_argument("--batch-only", action="store_true", help="run the batched leg alone")
+    ap.add_argument("--direct-only", action="store_true",
+                    help="run the direct-method leg alone (1024 ch x 2^18, DM 0-1000), "
+                         "GPU only, --steps executes")
     ap.add_argument("--served-only", action="store_true",
                     help="run the served leg alone (no plain-call line)")
     ap.add_argument("--served-chunks", type=int, default=6,
@@ -101,6 +105,30 @@ def make_workload(threads=8):
     return x, spec, info
 
 
+def host_cpu():
+    """Threads for the CPU baseline (every core this process may run on, as
+    nproc counts them, within the cgroup's CPU quota when one is set) and the
+    CPU model."""
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = max(1, int(math.ceil(int(q) / int(per))))
+    except (OSError, ValueError):
+        pass
+    threads = min(cores, quota) if quota else cores
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"threads": int(threads), "nproc": int(cores), "cgroup_quota": quota, "model": model}
+
+
 def cpu_baseline(x_host: np.ndarray, info, target_s: float):
     """The reference CPU path (cpucal direct loop nest, OpenMP) timed on a DM
     subset of the same chunk; returns the rate in DM-trial*samples/s."""
@@ -110,8 +138,9 @@ def cpu_baseline(x_host: np.ndarray, info, target_s: float):
     band = synth.band_1000_1500(info["nchan"])
     f = band.freq
     nchan, nout = info["nchan"], info["nout"]
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-    threads = max(1, min(threads, 16))
+    hc = host_cpu()
+    threads = hc["threads"]
+    native = O.cpucal_native()     # built on this host: -Ofast -march=native -fopenmp
     all_dms = np.arange(info["ndm"]) * 0.5
 
     def run(k):
@@ -119,7 +148,7 @@ def cpu_baseline(x_host: np.ndarray, info, target_s: float):
         # cpucal excludes the top channel: channels [chan_start, chan_end)
         d = O.delay_table(f, 0, nchan - 1, f[-1], float(band.tsamp), 1, all_dms[idx])
         t0 = time.perf_counter()
-        O.cpucal_direct_u8(x_host, 0, nchan - 1, d, 0, nout, 1, nthreads=threads)
+        O.cpucal_direct_u8(x_host, 0, nchan - 1, d, 0, nout, 1, nthreads=threads, lib=native)
         return time.perf_counter() - t0
 
     k = 8
@@ -133,10 +162,13 @@ def cpu_baseline(x_host: np.ndarray, info, target_s: float):
     med = float(np.median(reps))
     rate = k2 * nout / med
     return {"value": rate, "unit": UNIT, "cores": threads, "kind": "port",
+            "cpu_model": hc["model"], "nproc": hc["nproc"], "cgroup_cpu_quota": hc["cgroup_quota"],
+            "build": "gcc -Ofast -march=native -fopenmp on this host (CMakeLists.txt:118-119 flags)",
             "sample": f"{k2} of {info['ndm']} DM trials (evenly spaced over 0-3000) x {nout} "
                       f"samples x {nchan - 1} channels, cpucal direct loop nest "
-                      f"(include/cpucal.hpp:167-186), u64 accumulation, median of {len(reps)} runs, "
-                      f"{med:.2f} s each"}
+                      f"(include/cpucal.hpp:167-186), u64 accumulation, {threads} OpenMP threads, "
+                      f"median of {len(reps)} runs, {med:.2f} s each; rate scaled linearly to "
+                      f"all {info['ndm']} trials"}
 
 
 def served_leg(threads, chunks_per_beam, full_rfi=False):
@@ -394,12 +426,15 @@ def direct_leg(threads, cpu_target_s, reps=10, with_cpu=True):
         from oracle import oracle as O   # CPU baseline leg: the oracle's cpucal restatement
         # cpucal's channel range excludes the top channel: [0, nchan - 1)
         all_dms = np.arange(1001, dtype=np.float64)
+        hc = host_cpu()
+        native = O.cpucal_native()
 
         def run(k):
             idx = np.linspace(0, 1000, k).round().astype(np.int64)
             d = O.delay_table(f, 0, nchan - 1, f[-1], float(band.tsamp), 1, all_dms[idx])
             t = time.perf_counter()
-            O.cpucal_direct_u8(x, 0, nchan - 1, d, 0, nsamp, 1, nthreads=threads)
+            O.cpucal_direct_u8(x, 0, nchan - 1, d, 0, nsamp, 1, nthreads=hc["threads"],
+                               lib=native)
             return time.perf_counter() - t
 
         dt = run(4)
@@ -407,10 +442,11 @@ def direct_leg(threads, cpu_target_s, reps=10, with_cpu=True):
         el = run(k)
         cpu_rate = k * nsamp / el
         res["cpu_baseline"] = {
-            "value": cpu_rate, "unit": UNIT, "cores": threads, "kind": "port",
+            "value": cpu_rate, "unit": UNIT, "cores": hc["threads"], "kind": "port",
+            "cpu_model": hc["model"],
             "sample": f"{k} of 1001 DM trials (evenly spaced) x {nsamp} samples x {nchan - 1} "
                       f"channels, cpucal direct loop nest (include/cpucal.hpp:167-186), "
-                      f"{el:.2f} s"}
+                      f"-Ofast -march=native -fopenmp, {el:.2f} s"}
         res["speedup_vs_cpu"] = res["rate"] / cpu_rate
     return res
 
@@ -486,6 +522,10 @@ def main():
         sys.exit(1)
     capi.check(capi.lib().af_set_device(0))
     threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)))
+    if a.direct_only:
+        print(json.dumps({"direct_1024": direct_leg(threads, 0.0, reps=max(1, a.steps),
+                                                    with_cpu=False)}))
+        return
     if a.served_only:
         print(json.dumps({"served": served_leg(threads, a.served_chunks, a.served_full_rfi)}))
         return
@@ -539,14 +579,14 @@ def main():
         capi.sync()
         call_ms.append((time.perf_counter() - t0) * 1e3)
 
+    # B_alg: the input window read once plus the plane written once
+    # (nchan * (N_out * ds + D_max) * sizeof(T_in) + N_DM * N_out * sizeof(T_out))
     b_in = info["nchan"] * (info["nout"] * info["ds"] + info["dmax"]) * 1
     b_out = info["ndm"] * info["nout"] * 4
     b_alg = b_in + b_out
-    # dominant kernel: subband stage 1.  Its algorithmic bytes are the cleaned
-    # input window read once plus the subband partial sums written once.
+    # dominant kernel: subband stage 1; achieved = B_alg / its average time
     dom_ms = dom["dedisp"]
-    b_stage1 = b_in + plan.info.inter_bytes
-    achieved = b_stage1 / (dom_ms / 1e3) / 1e9
+    achieved = b_alg / (dom_ms / 1e3) / 1e9
     adds = plan.info.adds
     traffic = profiled_traffic(STAGE1_KERNEL)
 
@@ -565,7 +605,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": STAGE1_KERNEL + " (subband stage 1)",
-                     "kernel_ms": dom_ms, "alg_bytes": b_stage1,
+                     "kernel_ms": dom_ms, "alg_bytes": b_alg,
+                     "alg_bytes_def": "B_alg = nchan*(N_out*ds + D_max)*1 B + N_DM*N_out*4 B "
+                                      "(input window read once + plane written once)",
+                     "path_frac": b_alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
                      "traffic_source": TRAFFIC_FILE if traffic else None},
         "path_roofline": {"alg_bytes": b_alg, "GB_s": b_alg / (ms / 1e3) / 1e9,
                           "hbm_frac": b_alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,

@@ -203,7 +203,9 @@ af_status af_time_execute(af_plan *plan, const void *d_in, void *d_out, int reps
  * workgroup finding its plan in a table.  Results equal af_execute of each
  * plan on its buffers.  The buffers are bound at creation; plans and
  * buffers must outlive the batch, and a plan must not run elsewhere while
- * the batch runs (they share the plan's scratch). */
+ * the batch runs (they share the plan's scratch).  Each entry needs a plan
+ * of its own: the same plan given twice is rejected (AF_ERR_INVALID), since
+ * both entries would write its scratch at once. */
 typedef struct af_batch af_batch;
 
 /* d_masks may be NULL (or hold NULL entries). */
@@ -217,13 +219,21 @@ af_status af_batch_info(const af_batch *batch, int *launches, int *recorded);
 af_status af_batch_destroy(af_batch *batch);
 
 /* ----------------------------------------------------------- autotuner --
- * Times the result-invariant kernel choices of a plan on the caller's
- * buffers -- stage-1 DM rows per workgroup {4, 16} and stage-2 output tile
- * {256, 512, 1024} -- and, when tune_subbands != 0, the subband width (16,
- * 32, 64, 128 channels: this changes the method's rounding of delays and
- * therefore the plane).  Writes the fastest settings into *desc (tile_t,
- * dm_per_block, sb_nchan) and returns the best time per execute.  Results
- * are kept per shape for the life of the process. */
+ * Replaces the reference's compile-time knobs AF_SUBBAND_SIZE_CH,
+ * AF_SUBBAND_NDM0 and AF_SUBBAND_TBLOCK (src/core/dedispered.cu:25-39) with
+ * a measured choice.  Times candidate plans on the caller's buffers:
+ *  - always: the stage-1 form for u8 (dm_per_block 0 = 8 DM rows per
+ *    workgroup loading their windows through the L1, 16 = 16 rows sharing
+ *    an LDS window) and the stage-2 output tile (0 = the plan's own choice,
+ *    256, 512, 1024 for u8; 0, 512, 1024 for float32).  These do not change
+ *    the plane (tested);
+ *  - when tune_subbands != 0, also the subband layout: sb_nchan {16, 32,
+ *    64, 128} x ndm0 {8, 16, 32, 64}.  This changes the method's rounding
+ *    of delays and therefore the plane.
+ * Writes the fastest settings into *desc (tile_t, dm_per_block, sb_nchan,
+ * ndm0) and returns the best time per execute.  Results are kept for the
+ * life of the process, keyed by every field of *desc except the tuned ones,
+ * the frequency table, the static mask and the device. */
 af_status af_autotune(af_plan_desc *desc, const void *d_in, void *d_out, int tune_subbands,
                       int reps, float *best_ms);
 
@@ -283,7 +293,7 @@ af_status af_server_destroy(af_server *srv);
  * tail rows; window->nout is the chunk's output length (ds must be 1).
  * chunk_rows new rows arrive per chunk (chunk_rows = window->nout).  The
  * beam's buffers count against the budget (AF_ERR_NOMEM when they do not
- * fit).  Not allowed while chunks are in flight. */
+ * fit).  Not allowed while chunks are in flight (AF_ERR_BUSY). */
 af_status af_server_add_beam(af_server *srv, const af_plan_desc *window, size_t chunk_rows,
                              int *beam);
 /* Queues the beam's next chunk_rows rows ([chunk_rows][nchan], time-major).
@@ -296,7 +306,9 @@ af_status af_server_submit(af_server *srv, int beam, const void *h_rows, uint64_
 /* Returns up to max finished chunks, in completion order; waits up to
  * wait_ms for the first one. */
 af_status af_server_poll(af_server *srv, af_server_result *out, int max, int *n, int wait_ms);
-/* Gives a result plane (and its masks) back to the pool. */
+/* Gives a result plane (and its masks) back to the pool.  Only a plane that
+ * af_server_poll returned and that has not been released since is accepted
+ * (AF_ERR_INVALID otherwise). */
 af_status af_server_release(af_server *srv, void *d_plane);
 /* Waits until every submitted chunk has finished. */
 af_status af_server_drain(af_server *srv);

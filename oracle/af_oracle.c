@@ -136,6 +136,9 @@ void afo_residual_table(const double *freq, int c0, int c1, int sbw,
 /* src/core/dedispered.cu:337-376 with USE_SQRT_COMPRESSION. */
 void afo_bin(const void *in, int dtype, size_t nsamp, int nchan, int ds, void *out) {
   size_t nout = (nsamp + ds - 1) / ds;
+  /* rows are independent: split over threads (each element's arithmetic is
+   * unchanged) */
+#pragma omp parallel for schedule(static)
   for (size_t s = 0; s < nout; ++s) {
     size_t t0 = s * (size_t)ds;
     size_t t1 = t0 + (size_t)ds < nsamp ? t0 + (size_t)ds : nsamp;
@@ -216,9 +219,19 @@ void afo_iqrm_stat(const void *data, int dtype, int nchan, uint32_t t0,
                    uint32_t t1, int c0, int c1, int mode, int flavour,
                    float *x) {
   /* Row-major traversal with per-channel accumulators: each channel still
-   * sums its samples in ascending time order. */
+   * sums its samples in ascending time order.  Large windows split their
+   * channels into blocks over threads (the same order per channel). */
   const uint32_t W = t1 - t0;
   const int C = c1 - c0;
+  const int nblk = (C + 255) / 256;
+  if (nblk > 1 && (uint64_t)W * (uint64_t)C > (1u << 22)) {
+#pragma omp parallel for schedule(dynamic)
+    for (int b = 0; b < nblk; ++b) {
+      const int a0 = c0 + 256 * b, a1 = a0 + 256 < c1 ? a0 + 256 : c1;
+      afo_iqrm_stat(data, dtype, nchan, t0, t1, a0, a1, mode, flavour, x + (a0 - c0));
+    }
+    return;
+  }
   double *ds = (double *)calloc((size_t)C, sizeof(double));
   double *dq = (double *)calloc((size_t)C, sizeof(double));
   uint64_t *us = (uint64_t *)calloc((size_t)C, sizeof(uint64_t));
@@ -318,6 +331,33 @@ static float pct_gpu(const float *v, size_t m, float q) {
   return (float)r;
 }
 
+/* Quartiles and median of m lagged differences: the CPU flavour drops
+ * non-finite values and interpolates in float (include/iqrm.hpp:56-63,
+ * 118-123), the GPU flavour sorts all m and interpolates with a double
+ * position and weight (src/core/iqrm.cu:288-306).  Returns the number of
+ * values used (0: no quartiles). */
+size_t afo_iqrm_quartiles(const float *d, size_t m_in, int flavour, float *srt, float q[3]) {
+  size_t m = 0;
+  if (flavour == AFO_IQRM_CPU) {
+    for (size_t k = 0; k < m_in; ++k)
+      if (isfinite(d[k])) srt[m++] = d[k];
+    if (m == 0) return 0;
+    qsort(srt, m, sizeof(float), cmp_float);
+    q[0] = pct_cpu(srt, m, 0.25f);
+    q[1] = pct_cpu(srt, m, 0.50f);
+    q[2] = pct_cpu(srt, m, 0.75f);
+  } else {
+    memcpy(srt, d, sizeof(float) * m_in);
+    m = m_in;
+    if (m == 0) return 0;
+    qsort(srt, m, sizeof(float), cmp_float);
+    q[0] = pct_gpu(srt, m, 0.25f);
+    q[1] = pct_gpu(srt, m, 0.50f);
+    q[2] = pct_gpu(srt, m, 0.75f);
+  }
+  return m;
+}
+
 /* Partner channel of a flagged channel k for a lag
  * (src/core/iqrm.cu:138-145, include/iqrm.hpp:134-141). */
 static int partner(int k, int lag, int C) {
@@ -352,24 +392,9 @@ void afo_iqrm_on_stat(const float *x, int C, float radius_frac, float nsigma,
       float edge = x[lag] - x[0];
       for (int k = 0; k < C; ++k) d[k] = (k < lag) ? edge : x[k] - x[k - lag];
     }
-    float q1, med, q3;
-    size_t m = 0;
-    if (flavour == AFO_IQRM_CPU) {
-      for (int k = 0; k < C; ++k)
-        if (isfinite(d[k])) srt[m++] = d[k];
-      if (m == 0) continue;
-      qsort(srt, m, sizeof(float), cmp_float);
-      q1 = pct_cpu(srt, m, 0.25f);
-      med = pct_cpu(srt, m, 0.50f);
-      q3 = pct_cpu(srt, m, 0.75f);
-    } else {
-      memcpy(srt, d, sizeof(float) * C);
-      m = (size_t)C;
-      qsort(srt, m, sizeof(float), cmp_float);
-      q1 = pct_gpu(srt, m, 0.25f);
-      med = pct_gpu(srt, m, 0.50f);
-      q3 = pct_gpu(srt, m, 0.75f);
-    }
+    float q[3];
+    if (afo_iqrm_quartiles(d, (size_t)C, flavour, srt, q) == 0) continue;
+    const float q1 = q[0], med = q[1], q3 = q[2];
     float s = fabsf((q3 - q1) / 1.349f);
     if (!(s > 0.f) || !isfinite(s)) continue;
     float thr = nsigma * s;
@@ -435,6 +460,7 @@ void afo_zero_dm(void *data, int dtype, size_t nsample, int nchan, int c0, int c
   if (c0 < 0) c0 = 0;
   if (c1 > nchan) c1 = nchan;
   if (c0 >= c1) return;
+#pragma omp parallel for schedule(static)
   for (size_t t = 0; t < nsample; ++t) {
     size_t row = t * (size_t)nchan;
     if (dtype == AFO_F32) {
@@ -482,9 +508,17 @@ void afo_dedisperse_direct(const void *data, int dtype, size_t nsamp, int nchan,
                            int c0, int c1, const int32_t *delays, int ndm,
                            size_t nout, void *out) {
   int nc = c1 - c0;
-  for (int i = 0; i < ndm; ++i) {
+  /* (DM row, block of 4096 outputs) pairs over threads; every output is
+   * summed in channel order as written */
+  const size_t nb = (nout + 4095) / 4096;
+  const long long npair = (long long)ndm * (long long)nb;
+#pragma omp parallel for schedule(dynamic)
+  for (long long q = 0; q < npair; ++q) {
+    const int i = (int)(q / (long long)nb);
+    const size_t tb = (size_t)(q % (long long)nb) * 4096;
+    const size_t te = tb + 4096 < nout ? tb + 4096 : nout;
     const int32_t *d = delays + (size_t)i * nc;
-    for (size_t t = 0; t < nout; ++t) {
+    for (size_t t = tb; t < te; ++t) {
       if (dtype == AFO_F32) {
         double acc = 0.0;
         for (int c = c0; c < c1; ++c) {
@@ -521,13 +555,21 @@ void afo_dedisperse_subband_rows(const void *data, int dtype, size_t nsamp, int 
   int nc = c1 - c0;
   int nsb = (nc + sbw - 1) / sbw;
   (void)ndm;
-  for (int ri = 0; ri < nrows; ++ri) {
+  /* (row, block of 4096 outputs) pairs over threads; every output is summed
+   * subband by subband, each subband in channel order, as written */
+  const size_t nb = (nout + 4095) / 4096;
+  const long long npair = (long long)nrows * (long long)nb;
+#pragma omp parallel for schedule(dynamic)
+  for (long long q = 0; q < npair; ++q) {
+    const int ri = (int)(q / (long long)nb);
+    const size_t tb = (size_t)(q % (long long)nb) * 4096;
+    const size_t te = tb + 4096 < nout ? tb + 4096 : nout;
     int m = rows ? rows[ri] : ri;
     size_t orow = (size_t)ri;
     int j = m / ndm0;
     const int32_t *d1 = delays1 + (size_t)j * nc;
     const int32_t *r = res + (size_t)m * nsb;
-    for (size_t t = 0; t < nout; ++t) {
+    for (size_t t = tb; t < te; ++t) {
       uint64_t acc = 0;
       double accf = 0.0;
       for (int sb = 0; sb < nsb; ++sb) {

@@ -30,6 +30,18 @@
  *   afo_dedisperse_subband                        : equal to direct when subband width=1 and
  *                                                   NDM0=1; brute force on tiny inputs
  *   afo_cpucal_direct                             : equal to afo_dedisperse_direct for ds=1
+ *   afo_iqrm_quartiles                            : closed forms at every C mod 4 (exact
+ *                                                   interpolation weights 0, 1/4, 1/2, 3/4),
+ *                                                   and a case where float and double
+ *                                                   interpolation round differently
+ *
+ * Floating-point contraction.  Every C file here is compiled with
+ * -ffp-contract=off: a*b+c rounds twice, as written.  "Bit-exact to the
+ * reference" for the floating-point steps (IQRM statistic Sum x^2/W - mean^2
+ * and percentile interpolation, zero-DM mean and subtraction, delay tables)
+ * rests on that reading of the source text; nvcc's default (--fmad=true)
+ * may contract such expressions into fused multiply-adds, which would change
+ * the last bit of a statistic in rare cases.
  */
 #ifndef AF_ORACLE_H
 #define AF_ORACLE_H
@@ -105,6 +117,11 @@ void afo_iqrm_stat(const void *data, int dtype, int nchan, uint32_t t0,
 
 /* Robust lag voting on a statistic vector, src/core/iqrm.cu:244-327 and
  * include/iqrm.hpp:102-159.  mask[C] gets 0/1. */
+/* Quartiles q[0..2] = (q1, median, q3) of m lagged differences, sorted into
+ * srt[m]: flavour CPU (finite values, float interpolation) or GPU (all
+ * values, double position and weight).  Returns the count used (0: none). */
+size_t afo_iqrm_quartiles(const float *d, size_t m, int flavour, float *srt, float q[3]);
+
 void afo_iqrm_on_stat(const float *x, int C, float radius_frac, float nsigma,
                       double geofactor, int flavour, uint8_t *mask);
 

@@ -58,6 +58,7 @@ def lib():
             "afo_iqrm_radius": (I, [F, I]),
             "afo_iqrm_stat": (None, [P, I, I, U, U, I, I, I, I, P]),
             "afo_iqrm_on_stat": (None, [P, I, F, F, D, I, P]),
+            "afo_iqrm_quartiles": (S, [P, S, I, P, P]),
             "afo_iqrm": (I, [P, I, S, I, I, I, I, F, F, D, U, U, I, I, P, P, P, I]),
             "afo_apply_window_masks": (None, [P, I, S, I, P, P, P, I]),
             "afo_zero_dm": (None, [P, I, S, I, I, I]),
@@ -150,6 +151,15 @@ def iqrm_stat(x, t0, t1, c0, c1, mode, flavour=IQRM_GPU):
     return out
 
 
+def iqrm_quartiles(d, flavour=IQRM_GPU):
+    """(q1, median, q3) of lagged differences d, as the IQRM vote takes them."""
+    d = np.ascontiguousarray(d, np.float32)
+    srt = np.empty(max(1, len(d)), np.float32)
+    q = np.zeros(3, np.float32)
+    n = lib().afo_iqrm_quartiles(_p(d), len(d), flavour, _p(srt), _p(q))
+    return (q if n else None), n
+
+
 def iqrm_on_stat(stat, radius_frac, nsigma, geofactor, flavour=IQRM_GPU):
     stat = np.ascontiguousarray(stat, np.float32)
     m = np.empty(len(stat), np.uint8)
@@ -239,14 +249,37 @@ def dedisperse_direct_rows(x, c0, c1, delays, nout, rows):
     return dedisperse_direct(x, c0, c1, np.ascontiguousarray(delays[rows]), nout)
 
 
-def cpucal_direct_u8(x, c0, c1, delays, start, length, ds, nthreads=0):
+_native = None
+
+
+def cpucal_native():
+    """af_cpucal.c compiled on the running host with the reference's host
+    flags (-Ofast -march=native -fopenmp, CMakeLists.txt:118-119) into a
+    temporary directory, so -march=native means this machine's CPU."""
+    global _native
+    if _native is None:
+        import subprocess
+        import tempfile
+        src = os.path.join(_HERE, "af_cpucal.c")
+        out = os.path.join(tempfile.mkdtemp(prefix="af_cpucal_"), "libcpucal_native.so")
+        subprocess.run(["gcc", "-Ofast", "-march=native", "-fopenmp", "-fPIC", "-shared",
+                        "-std=c11", f"-I{_HERE}", src, "-o", out], check=True)
+        _native = C.CDLL(out)
+        f = _native.afo_cpucal_direct_u8
+        f.restype = None
+        P, I, S = C.c_void_p, C.c_int, C.c_size_t
+        f.argtypes = [P, S, I, I, I, P, I, S, S, I, P, I]
+    return _native
+
+
+def cpucal_direct_u8(x, c0, c1, delays, start, length, ds, nthreads=0, lib=None):
     x = np.ascontiguousarray(x, np.uint8)
     n, nc = x.shape
     delays = np.ascontiguousarray(delays, np.int32)
     ndm = delays.shape[0]
     out = np.empty((ndm, (length + ds - 1) // ds), np.uint64)
-    lib().afo_cpucal_direct_u8(_p(x), n, nc, c0, c1, _p(delays), ndm, start, length, ds,
-                               _p(out), nthreads)
+    (lib or globals()["lib"]()).afo_cpucal_direct_u8(_p(x), n, nc, c0, c1, _p(delays), ndm,
+                                                     start, length, ds, _p(out), nthreads)
     return out
 
 
@@ -323,12 +356,14 @@ def run_pipeline(x: np.ndarray, spec: PlanSpec, flavour=IQRM_GPU):
     return plane, masks, t0s, t1s, x
 
 
-def run_pipeline_rows(x: np.ndarray, spec: PlanSpec, rows, flavour=IQRM_GPU):
+def run_pipeline_rows(x: np.ndarray, spec: PlanSpec, rows, flavour=IQRM_GPU, binned=False):
     """run_pipeline for the DM rows `rows` only (large inputs).  Returns
-    (plane rows, masks, cleaned)."""
-    x = np.array(x, copy=True, order="C")
-    if spec.ds > 1:
-        x = bin_time(x, spec.ds)
+    (plane rows, masks, cleaned).  binned=True: x is already at the binned
+    rate (the binning step is skipped; the tables still use spec.ds)."""
+    if spec.ds > 1 and not binned:
+        x = bin_time(x, spec.ds)      # a new array
+    else:
+        x = np.array(x, copy=True, order="C")
     nbin = x.shape[0]
     nout = spec.nout or nbin      # 0 or None: the full binned length
     masks = np.zeros((0, spec.nchan), np.uint8)

@@ -123,3 +123,33 @@ def test_flat_slice_maps_to_the_first_colour():
     # where resizing is exact)
     img = core._image_slice(np.full((512, 512), 7.0, np.float32), 512)
     assert np.all(img == _viridis_bgr()[0])
+
+
+def _rfi_iqrm():
+    return core.RFIconfig(False, True, True, core.IQRMconfig(1, 0.1, 3.0, 1.5, 0.0, 0.0, True))
+
+
+def test_derived_description_follows_the_reference():
+    # 64 channels 1000.0 + 0.5 i MHz: freq 1004-1020 maps to channels 8..40
+    # (src/core/dedispered.cu:1017-1022); IQRM and zero-DM take [cs, ce)
+    h = core.Header(0.0, 1e-4, 1000.0, 0.5, 8, 64, 1000, "x")
+    d = core._describe_spec(1000, h, 0.0, 10.0, 1004.0, 1020.0, 0.5, 2, "", _rfi_iqrm())
+    assert (d["chan_start"], d["chan_end"]) == (8, 40)
+    assert (d["dd_c0"], d["dd_c1"]) == (8, 41)
+    assert (d["use_iqrm"], d["iqrm_c0"], d["iqrm_c1"]) == (1, 8, 40)
+    assert (d["use_zero_dm"], d["zdm_c0"], d["zdm_c1"]) == (1, 8, 40)
+    assert d["dm_steps"] == 21 and d["down_ndata"] == 500
+    assert d["f_ref_mhz"] == pytest.approx(1020.0)
+
+
+def test_single_channel_range_skips_iqrm():
+    # freq_start == freq_end == the top channel: cs == ce, so IQRM's channel
+    # range [cs, ce) is empty and the reference computes no window masks
+    # (src/core/iqrm.cu:363-364); the call goes ahead without IQRM
+    h = core.Header(0.0, 1e-4, 1000.0, 0.5, 8, 64, 1000, "x")
+    top = 1000.0 + 0.5 * 63
+    d = core._describe_spec(1000, h, 0.0, 10.0, top, top, 1.0, 1, "", _rfi_iqrm())
+    assert d["chan_start"] == d["chan_end"] == 63
+    assert (d["dd_c0"], d["dd_c1"]) == (63, 64)
+    assert d["use_iqrm"] == 0
+    assert d["zdm_c0"] == d["zdm_c1"] == 63

@@ -73,3 +73,18 @@ def test_plan_create_without_gpu_reports_nodev():
     spec = make_spec(x, small_band(8), 0.0, 1.0, 4, method="direct")
     with pytest.raises(capi.AFError, match="No HIP devices"):
         capi.Plan(spec)
+
+
+def test_batch_rejects_the_same_plan_twice():
+    # the duplicate check precedes any use of the plans: opaque non-null
+    # handles are enough to reach it without a GPU
+    lib = capi.lib()
+    token = C.create_string_buffer(64)
+    h = C.cast(token, C.c_void_p).value
+    bufs = (C.c_void_p * 2)(C.c_void_p(h), C.c_void_p(h))
+    plans = (C.c_void_p * 2)(C.c_void_p(h), C.c_void_p(h))
+    out = C.c_void_p()
+    rc = lib.af_batch_create(plans, 2, bufs, bufs, None, C.byref(out))
+    assert rc == capi.AF_ERR_INVALID
+    assert b"more than once" in lib.af_last_error()
+    assert not out.value

@@ -148,14 +148,20 @@ def test_subband_u8_wide_subbands_generic(gpu_lib):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("ds", [1, 2, 4])
 @pytest.mark.parametrize("dtype", [np.uint16, np.uint32])
 @pytest.mark.parametrize("method", ["direct", "subband"])
-def test_wide_integer_types(gpu_lib, dtype, method):
-    say(f"case: {np.dtype(dtype).name} {method}")
-    rng = np.random.default_rng(7)
+def test_wide_integer_types(gpu_lib, dtype, method, ds):
+    # ds > 1 runs the binning kernel on 16/32-bit data (sqrt compression with
+    # the type's own vmax, src/core/dedispered.cu:337-376) before IQRM
+    say(f"case: {np.dtype(dtype).name} {method} ds {ds}")
+    rng = np.random.default_rng(7 + ds)
     b = small_band(200, 1200.0, 1500.0, 1e-4)
-    x = noise(rng, 3000, 200, dtype)
-    spec = make_spec(x, b, 0.0, 5.0, 33, method=method, sbw=25, ndm0=8,
+    x = noise(rng, 3001, 200, dtype)
+    if dtype == np.uint32:
+        # values near the top of the range: the binned sums exceed 2^32
+        x[::7] = np.uint32(4294967295 - 5)
+    spec = make_spec(x, b, 0.0, 5.0, 33, method=method, sbw=25, ndm0=8, ds=ds,
                      use_iqrm=True, iqrm_c0=0, iqrm_c1=199, iqrm_mode=1,
                      use_zero_dm=True, zdm_c0=0, zdm_c1=199)
     got, ref, masks, rmask = run_both(x, spec)

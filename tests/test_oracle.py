@@ -366,3 +366,81 @@ def test_cpucal_restatement_equals_direct():
     # decimation for ds > 1: reads native samples ti*ds + d
     dec = O.cpucal_direct_u8(x, cs, ce, dd, 0, 500, 4, nthreads=2)
     assert np.array_equal(dec, bb[:, ::4].astype(np.uint64))
+
+
+# ------------------------------------------- IQRM quartiles, closed forms --
+
+@pytest.mark.parametrize("C_", [4093, 4094, 4095, 4096, 41, 42, 43, 44])
+@pytest.mark.parametrize("flavour", [O.IQRM_GPU, O.IQRM_CPU])
+def test_iqrm_quartiles_closed_form(C_, flavour):
+    # d = 2*k in shuffled order: the sorted values are v[k] = 2k, so the
+    # quantile at position p = q*(C-1) is exactly 2p for every C mod 4
+    # (weights 0, 1/4, 1/2, 3/4).  Wrong position (q*C), no interpolation,
+    # or nearest rank all give other values.
+    rng = np.random.default_rng(C_)
+    d = (2.0 * rng.permutation(C_)).astype(np.float32)
+    q, n = O.iqrm_quartiles(d, flavour)
+    assert n == C_
+    assert q.tolist() == [0.5 * (C_ - 1), 1.0 * (C_ - 1), 1.5 * (C_ - 1)]
+
+
+def _round_f32(fr):
+    """Single rounding of an exact rational to float32 (via an exact double:
+    the values here need fewer than 53 bits)."""
+    from fractions import Fraction
+    x = float(fr)
+    assert Fraction(x) == fr
+    return np.float32(x)
+
+
+def test_iqrm_gpu_quartile_rounds_once_in_double():
+    # C = 4096: q1 sits at position 1023.75, a weight of 3/4 between the
+    # sorted values 1023 and 1024.  With lo = -1.6023048, hi = 1.779193 the
+    # float path rounds diff*0.75 and then the sum (two roundings) while the
+    # GPU flavour's double position and weight give one rounding of the
+    # exact lo + fl(hi - lo) * 3/4 (src/core/iqrm.cu:299-306).  The two
+    # differ in the last bit here.
+    from fractions import Fraction
+    lo, hi = np.float32(-1.6023048), np.float32(1.779193)
+    C_ = 4096
+    v = np.concatenate([np.linspace(-50, -10, 1023).astype(np.float32), [lo, hi],
+                        np.linspace(10, 50, C_ - 1025).astype(np.float32)])
+    d = v[np.random.default_rng(1).permutation(C_)]
+    diff = np.float32(hi - lo)
+    exact = _round_f32(Fraction(float(lo)) + Fraction(float(diff)) * Fraction(3, 4))
+    twice = np.float32(lo + np.float32(diff * np.float32(0.75)))
+    assert exact != twice
+    qg, _ = O.iqrm_quartiles(d, O.IQRM_GPU)
+    qc, _ = O.iqrm_quartiles(d, O.IQRM_CPU)
+    assert qg[0] == exact
+    assert qc[0] == twice
+
+
+def test_iqrm_gpu_stat_is_one_pass_std():
+    # A u32 channel alternating a and a + 2 (a = 4e9, W = 6 rows) has std 1
+    # exactly.  The two-pass form (CPU flavour, include/iqrm.hpp:207-214)
+    # gives 1.  The one-pass form of the GPU flavour, S2/W - mean^2 in double
+    # (src/core/iqrm.cu:83-89), cancels catastrophically: S2 ~ 9.6e19 needs
+    # 67 bits, so the difference is a multiple of double's spacing near
+    # 1.6e19 (2048), and the statistic comes out as sqrt(2048) = 45.25.
+    # Expected values: each form evaluated with Python floats (IEEE double,
+    # one rounding per operation, as the source text writes it).
+    import math
+    a, W = 4000000000, 6
+    col = np.array([a, a + 2] * (W // 2), np.uint32)
+    x = np.zeros((W, 2), np.uint32)
+    x[:, 0] = col
+    x[:, 1] = 7
+    S = sum(int(v) for v in col)
+    S2 = sum(int(v) * int(v) for v in col)
+    mean = float(S) / W
+    one = np.float32(math.sqrt(max(0.0, float(S2) / W - mean * mean)))
+    acc = 0.0
+    for v in col:
+        acc += (float(v) - mean) * (float(v) - mean)
+    two = np.float32(math.sqrt(acc / W))
+    assert two == 1.0 and one == np.float32(math.sqrt(2048.0))
+    g = O.iqrm_stat(x, 0, W, 0, 2, 1, O.IQRM_GPU)
+    c = O.iqrm_stat(x, 0, W, 0, 2, 1, O.IQRM_CPU)
+    assert g[0] == one and c[0] == two
+    assert g[1] == 0.0 and c[1] == 0.0
