@@ -121,6 +121,7 @@ SIGNATURES = {
     "af_stream_sync": (_I, [_P]),
     "af_sync": (_I, []),
     "af_time_execute": (_I, [_P, _P, _P, _I, C.POINTER(C.c_float)]),
+    "af_execute_noalloc": (_I, [C.POINTER(PlanDesc), _P, _P, _P, _P]),
     "af_batch_create": (_I, [C.POINTER(_P), _I, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
                              C.POINTER(_P)]),
     "af_batch_execute": (_I, [_P, _I, _P]),
@@ -250,6 +251,16 @@ class DedispSpec:
         d.inter_budget = self.inter_budget
         d.clean_budget = self.clean_budget
         return d, (freq, bad)
+
+
+def execute_noalloc(spec: DedispSpec, d_in: int, d_out: int, d_masks: int = 0, stream: int = 0):
+    """af_execute_noalloc: plan and execute in one call, every device buffer
+    taken from the library's static scratch (no allocation)."""
+    desc, keep = spec.to_desc()
+    check(lib().af_execute_noalloc(C.byref(desc), C.c_void_p(d_in), C.c_void_p(d_out),
+                                   C.c_void_p(d_masks) if d_masks else None,
+                                   C.c_void_p(stream) if stream else None), "af_execute_noalloc")
+    del keep
 
 
 class Plan:

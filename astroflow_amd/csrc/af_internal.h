@@ -324,6 +324,13 @@ struct af_plan {
   double acc_total = 0;
   int acc_n = 0;
   std::vector<void *> allocs;
+  // Arena mode (af_execute_noalloc): every device buffer is carved from
+  // [arena, arena + arena_cap) and the tables are copied in on arena_st;
+  // with arena_measure set nothing is copied and only arena_used is counted.
+  uint8_t *arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
+  bool arena_measure = false;
+  hipStream_t arena_st = nullptr;
   // af_execute_host: device buffers kept between calls (grow-only) and two
   // page-locked staging buffers with their events
   void *hx_in = nullptr, *hx_out = nullptr, *hx_masks = nullptr;

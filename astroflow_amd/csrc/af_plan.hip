@@ -92,10 +92,32 @@ static std::vector<int> make_lags(int radius, double geofactor) {
   return lags;
 }
 
+// Arena mode: the next 256-byte aligned piece of the arena.
+static af_status carve(af_plan *p, void **dst, size_t bytes) {
+  const size_t off = (p->arena_used + 255) / 256 * 256;
+  if (off + bytes > p->arena_cap && !p->arena_measure)
+    return fail(AF_ERR_NOMEM, "the plan needs more device scratch than the no-allocation "
+                              "entry point holds");
+  p->arena_used = off + bytes;
+  *dst = p->arena_measure ? nullptr : (void *)(p->arena + off);
+  return AF_OK;
+}
+
 template <typename V>
 static af_status upload(af_plan *p, V **dst, const std::vector<V> &src) {
   if (src.empty()) {
     *dst = nullptr;
+    return AF_OK;
+  }
+  if (p->arena || p->arena_measure) {
+    void *q = nullptr;
+    AF_TRY(carve(p, &q, src.size() * sizeof(V)));
+    // pageable source: the runtime copies it out before returning; the
+    // device copy is ordered on the caller's stream
+    if (q)
+      AF_HIP(hipMemcpyAsync(q, src.data(), src.size() * sizeof(V), hipMemcpyHostToDevice,
+                            p->arena_st));
+    *dst = (V *)q;
     return AF_OK;
   }
   void *q = nullptr;
@@ -113,6 +135,7 @@ static af_status dalloc(af_plan *p, void **dst, size_t bytes) {
     *dst = nullptr;
     return AF_OK;
   }
+  if (p->arena || p->arena_measure) return carve(p, dst, bytes);
   void *q = nullptr;
   hipError_t e = hipMalloc(&q, bytes);
   if (e != hipSuccess)
@@ -565,7 +588,16 @@ static af_status build(af_plan *p) {
   const size_t es = dtype_size(d.dtype);
   if (ds > 1) AF_TRY(dalloc(p, &p->d_xb, p->nbin * (size_t)d.nchan * es));
   AF_TRY(dalloc(p, &p->d_xT, (size_t)p->nch * p->Lp * es));
-  AF_HIP(hipMemset(p->d_xT, 0, (size_t)p->nch * p->Lp * es));
+  if (p->arena) {
+    // the arena keeps the previous call's contents: zero the columns past the
+    // input (the clean pass writes columns [0, nbin) of every row) with a
+    // kernel on the caller's stream
+    if (p->Lp > p->nbin)
+      AF_HIP(launch_zero_rows((uint8_t *)p->d_xT + p->nbin * es, p->Lp * es,
+                              (p->Lp - p->nbin) * es, p->nch, p->arena_st));
+  } else if (!p->arena_measure) {
+    AF_HIP(hipMemset(p->d_xT, 0, (size_t)p->nch * p->Lp * es));
+  }
   if (d.method == AF_METHOD_SUBBAND) {
     AF_TRY(dalloc(p, &p->d_inter, (size_t)p->ndm_nom * p->nsb * p->L1 * p->inter_elem));
   }
@@ -586,7 +618,8 @@ static af_status build(af_plan *p) {
       p->d_segmask = (uint8_t *)q;
     }
   }
-  AF_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+  if (!p->arena && !p->arena_measure)
+    AF_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
   return AF_OK;
 }
 
@@ -938,6 +971,34 @@ af_status af_execute_spectrum(af_plan *p, const void *d_in, void *d_out, size_t 
 
 namespace {
 
+// ------------------------------------------------ no-allocation scratch --
+// af_execute_noalloc carves every table and buffer of its plan from this
+// array.  Each device holds its own copy (allocated when the code object is
+// loaded on that device, kept until the process ends).  In host code the
+// name is not a device address: the address comes from hipGetSymbolAddress,
+// once per device, with that device current.
+constexpr size_t kNoAllocBytes = AF_NOALLOC_SCRATCH_BYTES;
+__device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t *g_scratch_ptr[kMaxDevices] = {};
+
+af_status scratch_of(int dev, uint8_t **ptr) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(AF_ERR_INVALID, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *q = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !q)
+      return fail(AF_ERR_RUNTIME, std::string("hipGetSymbolAddress(scratch): ") +
+                                      hipGetErrorString(e));
+    g_scratch_ptr[dev] = (uint8_t *)q;
+  }
+  *ptr = g_scratch_ptr[dev];
+  return AF_OK;
+}
+
 constexpr size_t kStageBytes = (size_t)32 << 20;   // per page-locked staging buffer
 
 // memcpy split over up to 8 host threads (pageable user memory <-> page-
@@ -1069,6 +1130,50 @@ af_status af_execute_host(af_plan *p, const void *h_in, void *h_out, uint8_t *h_
     return fail(AF_ERR_RUNTIME, std::string("D2H: ") + hipGetErrorString(e));
   }
   return AF_OK;
+}
+
+af_status af_execute_noalloc(const af_plan_desc *desc, const void *d_in, void *d_out,
+                            uint8_t *d_masks, void *stream) {
+  if (!desc || !d_in || !d_out) return fail(AF_ERR_INVALID, "null argument");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(AF_ERR_NODEV, "No HIP devices found");
+  AF_TRY(validate(*desc));
+  int dev = 0;
+  AF_HIP(hipGetDevice(&dev));
+  uint8_t *base = nullptr;
+  AF_TRY(scratch_of(dev, &base));
+  // the same plan af_plan_create builds, on host memory, with the whole
+  // channel-major buffer (no per-super-tile cleaning) and its device buffers
+  // carved from the scratch: first counted, then placed
+  auto make = [&](af_plan &p) {
+    p.d = *desc;
+    p.freq.assign(desc->freq_mhz, desc->freq_mhz + desc->nchan);
+    p.d.freq_mhz = nullptr;
+    if (desc->n_bad > 0) p.bad.assign(desc->bad_chans, desc->bad_chans + desc->n_bad);
+    p.d.bad_chans = nullptr;
+    p.d.clean_budget = ~(size_t)0;
+    p.device = dev;
+    p.arena_cap = kNoAllocBytes;
+    p.arena_st = (hipStream_t)stream;
+  };
+  size_t need = 0;
+  {
+    af_plan p;
+    make(p);
+    p.arena_measure = true;
+    AF_TRY(build(&p));
+    need = p.arena_used;
+  }
+  if (need > kNoAllocBytes)
+    return fail(AF_ERR_NOMEM, "af_execute_noalloc: the call needs " + std::to_string(need) +
+                                  " bytes of device scratch; the library holds " +
+                                  std::to_string(kNoAllocBytes));
+  af_plan p;
+  make(p);
+  p.arena = base;
+  AF_TRY(build(&p));
+  return execute(&p, d_in, d_out, d_masks, (hipStream_t)stream);
 }
 
 af_status af_time_execute(af_plan *p, const void *d_in, void *d_out, int reps, float *ms) {

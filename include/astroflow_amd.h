@@ -156,6 +156,25 @@ af_status af_execute(af_plan *plan, const void *d_in, void *d_out,
 af_status af_execute_spectrum(af_plan *plan, const void *d_in, void *d_out, size_t t0,
                               size_t t_len, uint8_t *d_masks, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  af_plan_create + af_execute + af_plan_destroy in one
+ * call that allocates nothing: the plan is built on host memory and all of
+ * its device tables and scratch buffers (cleaned channel-major buffer,
+ * subband intermediate, IQRM partials and masks) are carved from a static
+ * device array of AF_NOALLOC_SCRATCH_BYTES that the library holds on each
+ * device it runs on until the process ends.  Tables are copied in with
+ * hipMemcpyAsync on `stream` from host memory; stale contents are cleared
+ * by kernels on `stream`.  A call whose plan needs more than the array
+ * fails with AF_ERR_NOMEM before anything is launched.
+ * Every call on one device shares that device's array and leaves its last
+ * contents there: calls that overlap on one device (different streams or
+ * threads) race; calls in sequence on one stream do not.  Results equal
+ * af_execute of af_plan_create(desc) on the same buffers. */
+#define AF_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+af_status af_execute_noalloc(const af_plan_desc *desc, const void *d_in, void *d_out,
+                             uint8_t *d_masks, void *stream);
+
 /* Same on host buffers: upload, execute, download, synchronise.  The plan
  * keeps device buffers for the input and the plane from its first host call
  * on (grown when needed) and moves the data through two 32 MB page-locked
