@@ -107,6 +107,10 @@ struct CleanArgs {
   int zdm, zc0, zc1;
   int dd_c0, dd_c1;
   const int *run_if;        // u8 kernel: skip the launch when *run_if == 0 (null = always run)
+  // 1: item.pad holds how many channels from dd_c0 on are read by the
+  // dedispersion at the item's rows (their delays reach that far); later
+  // channels' columns are not stored there
+  int cut_in_pad;
 };
 hipError_t launch_clean(int dtype, const CleanArgs &a, hipStream_t st);
 
@@ -298,6 +302,7 @@ struct af_plan {
   // k*T2 + Lp) (channel-major, row pitch Lp); tile_items[tile_off[k] ..
   // tile_off[k+1]) are the clean items clipped to those rows
   bool seg_is_win = false;               // segment s is window s: segmask aliases the masks
+  bool clean_cut = false;                // clean_items[i].pad = channels read at its rows
   bool clean_tiled = false;
   std::vector<af::RowItem> tile_items;
   std::vector<int> tile_off;

@@ -587,11 +587,13 @@ __device__ __forceinline__ void clean_body(const CleanArgs &a, uint32_t bid, uin
     }
     __syncthreads();
   }
-  for (int cb = a.dd_c0; cb < a.dd_c1; cb += 64) {
+  // channels past the cut are never read at these rows (item.pad)
+  const int cend = a.cut_in_pad ? min(a.dd_c1, a.dd_c0 + (int)it.pad) : a.dd_c1;
+  for (int cb = a.dd_c0; cb < cend; cb += 64) {
     const int c = cb + lane;
     for (int rr = wave; rr < kCleanRows; rr += 4) {
       T v = 0;
-      if (c < a.dd_c1 && rr < nrows) {
+      if (c < cend && rr < nrows) {
         v = x[(size_t)(it.r0 + rr) * a.nchan + c];
         if (smask && smask[c]) v = 0;
         if (a.zdm && c >= a.zc0 && c < a.zc1) v = zdm_sub<T>(v, m_s[rr]);
@@ -602,7 +604,7 @@ __device__ __forceinline__ void clean_body(const CleanArgs &a, uint32_t bid, uin
     __syncthreads();
     for (int k = wave; k < 64; k += 4) {
       const int cc = cb + k;
-      if (cc < a.dd_c1 && lane < nrows)
+      if (cc < cend && lane < nrows)
         xT[(size_t)(cc - a.dd_c0) * a.Lp + it.r0 + lane] = tile[k][lane];
     }
     __syncthreads();

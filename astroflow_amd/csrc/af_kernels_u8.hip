@@ -425,6 +425,7 @@ __device__ __forceinline__ void clean_u8_body(const CleanArgs &a, uint32_t bid, 
     for (int b = 0; b < 4; ++b) {
       const int c = cb + 4 * k + b;
       if (c < a.dd_c0 || c >= a.dd_c1) continue;
+      if (a.cut_in_pad && c - a.dd_c0 >= (int)it.pad) continue;   // never read at these rows
       uint8_t *dst = xT + (size_t)(c - a.dd_c0) * a.Lp + it.r0 + rbase;
       if (full) {
         const uint2 v = make_uint2(lo[b], hi[b]);

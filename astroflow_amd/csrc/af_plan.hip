@@ -521,6 +521,29 @@ static af_status build(af_plan *p) {
       p->clean_items.push_back(RowItem{(uint32_t)s, r, std::min<uint32_t>(r + clean_rows, p->segs[s].t1), 0});
   }
 
+  // Columns of the cleaned buffer the dedispersion reads: channel c (from
+  // dd_c0) up to (nsuper-1)*T2 + L1 + its largest delay (+ the load
+  // overhang and the LDS window of the staged form).  Each clean item keeps
+  // in .pad the number of leading channels that still reach its first row;
+  // the clean pass stores only those (delays fall with frequency, so for the
+  // ascending band this is most of the low-frequency half at late rows).
+  if (!p->clean_tiled) {
+    const int rows = d.method == AF_METHOD_DIRECT ? p->ndm : p->ndm_nom;
+    std::vector<size_t> need((size_t)p->nch, 0);
+    const size_t base = (size_t)(p->nsuper - 1) * p->T2 + p->L1 + 64 + (size_t)p->dd_lds_win;
+    for (int c = 0; c < p->nch; ++c) {
+      int32_t mx = 0;
+      for (int r = 0; r < rows; ++r) mx = std::max(mx, p->h_delays[(size_t)r * p->nch + c]);
+      need[c] = base + (size_t)mx;
+    }
+    for (RowItem &it : p->clean_items) {
+      int cut = 0;
+      for (int c = p->nch - 1; c >= 0; --c)
+        if (need[c] > it.r0) { cut = c + 1; break; }
+      it.pad = (uint32_t)cut;
+    }
+    p->clean_cut = true;
+  }
   if (p->clean_tiled) {
     p->tile_off.assign(1, 0);
     for (int k = 0; k < p->nsuper; ++k) {
@@ -701,6 +724,7 @@ static af_status run_clean(af_plan *p, const void *d_in, hipStream_t st, int set
   ca.zc1 = d.zdm_c1;
   ca.dd_c0 = d.dd_c0;
   ca.dd_c1 = d.dd_c1;
+  ca.cut_in_pad = p->clean_cut ? 1 : 0;
   AF_HIP(launch_clean(d.dtype, ca, st));
   AF_TRY(record(p, set, 4, st));
   return AF_OK;
