@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_PKG, "libastroflow_amd.so")
 
 AF_OK, AF_ERR_INVALID, AF_ERR_RUNTIME, AF_ERR_NOMEM, AF_ERR_NODEV, AF_ERR_BUSY = range(6)
 AF_U8, AF_U16, AF_U32, AF_F32 = 1, 2, 3, 4
+AF_U64 = 5   # plane element type of u32 samples (af_image_slices)
 METHOD_DIRECT, METHOD_SUBBAND = 0, 1
 
 DTYPE_CODE = {np.dtype(np.uint8): AF_U8, np.dtype(np.uint16): AF_U16,
@@ -106,6 +107,8 @@ SIGNATURES = {
     "af_plan_stage_times": (_I, [_P, C.POINTER(StageTimes)]),
     "af_execute": (_I, [_P, _P, _P, _P, _P]),
     "af_execute_host": (_I, [_P, _P, _P, _P]),
+    "af_execute_host_images": (_I, [_P, _P, _S, _I, _P, _P]),
+    "af_image_slices": (_I, [_P, _I, _S, _S, _S, _S, _I, _P, _P]),
     "af_execute_spectrum": (_I, [_P, _P, _P, _S, _S, _P, _P]),
     "af_device_query": (_I, [_I, C.POINTER(DeviceProps)]),
     "af_device_count": (_I, [C.POINTER(_I)]),
@@ -333,6 +336,22 @@ class Plan:
                                     _ptr(masks) if masks is not None else None),
               "af_execute_host")
         return out, masks
+
+    def run_host_images(self, x: np.ndarray, sps: int, size: int = 512):
+        """The DM-time images of the plane (slices of sps columns, size x size
+        BGR), made on the device; the plane itself is not downloaded."""
+        x = np.ascontiguousarray(x)
+        if x.dtype != self.in_dtype:
+            raise ValueError(f"input dtype {x.dtype} != plan dtype {self.in_dtype}")
+        if x.size != self.spec.nsamp * self.spec.nchan:
+            raise ValueError("input size does not match the plan")
+        if sps <= 0:
+            raise ValueError("sps must be positive")
+        nsl = -(-self.info.nout // sps)
+        imgs = np.empty((nsl, size, size, 3), np.uint8)
+        check(lib().af_execute_host_images(self._h, _ptr(x), sps, size, _ptr(imgs), None),
+              "af_execute_host_images")
+        return imgs
 
     def execute(self, d_in: int, d_out: int, d_masks: int = 0, stream: int = 0):
         check(lib().af_execute(self._h, C.c_void_p(d_in), C.c_void_p(d_out),

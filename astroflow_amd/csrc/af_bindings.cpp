@@ -357,31 +357,13 @@ Raw run(py::array_t<T, py::array::c_style> data, const Header &h, float dm_low, 
   return r;
 }
 
-// Slices of t_sample seconds -> DM-time images (include/data.h:619-727).
-template <typename O>
-py::list images(const py::array_t<O> &plane, float tsamp, int ds, float t_sample) {
-  const size_t rows = (size_t)plane.shape(0), cols = (size_t)plane.shape(1);
+// Columns per image slice: t_sample seconds of (downsampled) samples
+// (include/data.h:619-727).
+size_t slice_cols(float tsamp, int ds, float t_sample) {
   const float tb = tsamp * (float)ds;
   const size_t sps = (size_t)(t_sample / tb);
   if (sps == 0) throw py::value_error("t_sample is shorter than one (downsampled) sample");
-  const size_t nslices = (cols + sps - 1) / sps;
-  constexpr int kSize = 512;
-  std::vector<std::vector<uint8_t>> bufs(nslices, std::vector<uint8_t>((size_t)kSize * kSize * 3));
-  const O *p = plane.data();
-  {
-    py::gil_scoped_release nogil;
-    for (size_t s = 0; s < nslices; ++s) {
-      const size_t c0 = s * sps, c1 = std::min(cols, c0 + sps);
-      afimg::slice_image(p, cols, c0, (int)rows, (int)(c1 - c0), kSize, bufs[s].data());
-    }
-  }
-  py::list out;
-  for (auto &b : bufs) {
-    py::array_t<uint8_t> a({kSize, kSize, 3});
-    std::memcpy(a.mutable_data(), b.data(), b.size());
-    out.append(a);
-  }
-  return out;
+  return sps;
 }
 
 template <typename T>
@@ -389,11 +371,42 @@ DedispersedDataUint8 dedisperse_spec(py::array_t<T, py::array::c_style> data, He
                                      float dm_low, float dm_high, float freq_start, float freq_end,
                                      float dm_step, int time_downsample, float t_sample,
                                      int target_id, std::string mask_file, RFIConfig rficfg) {
-  Raw r = run<T>(data, header, dm_low, dm_high, freq_start, freq_end, dm_step, time_downsample,
-                 target_id, mask_file, rficfg, false);
-  using O = typename OutOf<T>::type;
+  // the run() checks and plan, but the plane stays on the device: the
+  // images are made there and only they come back
+  const py::buffer_info bi = data.request();
+  if (bi.ndim < 1 || bi.ndim > 3) throw std::runtime_error("data must be 1D, 2D, or 3D");
+  if (header.ndata <= 0) throw py::value_error("header.ndata must be positive");
+  if ((size_t)bi.size < (size_t)header.ndata * (size_t)std::max(header.nchans, 0))
+    throw py::value_error("data holds fewer than header.ndata * header.nchans samples");
+  int ndev = 0;
+  if (af_device_count(&ndev) != AF_OK || ndev == 0) throw std::runtime_error("No HIP devices found");
+  if (target_id < 0 || target_id >= ndev) throw py::value_error("target_id names no device");
+  check(af_set_device(target_id), "af_set_device");
+  const size_t sps = slice_cols(header.tsamp, time_downsample, t_sample);
+  const Prepared prep = prepare(dtype_code<T>(), header, (size_t)header.ndata, dm_low, dm_high,
+                                freq_start, freq_end, dm_step, time_downsample, mask_file, rficfg);
+  PlanLease lease(prep, target_id);
+  af_plan_info info;
+  check(af_plan_query(lease.plan, &info), "af_plan_query");
+  constexpr int kSize = 512;
+  const size_t nslices = ((size_t)info.nout + sps - 1) / sps;
+  const size_t img_bytes = (size_t)kSize * kSize * 3;
+  std::vector<uint8_t> buf(nslices * img_bytes);
+  af_status st;
+  {
+    py::gil_scoped_release nogil;
+    st = af_execute_host_images(lease.plan, bi.ptr, sps, kSize, buf.data(), nullptr);
+  }
+  check(st, "af_execute_host_images");
+  lease.ok = true;
+  py::list images;
+  for (size_t s = 0; s < nslices; ++s) {
+    py::array_t<uint8_t> a({kSize, kSize, 3});
+    std::memcpy(a.mutable_data(), buf.data() + s * img_bytes, img_bytes);
+    images.append(a);
+  }
   DedispersedDataUint8 out;
-  out.dm_times = images<O>(py::array_t<O>(r.plane), header.tsamp, time_downsample, t_sample);
+  out.dm_times = images;
   out.shape = {512, 512, 3};
   out.dm_ndata = 512;
   out.downtsample_ndata = 512;

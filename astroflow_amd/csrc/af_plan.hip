@@ -150,7 +150,7 @@ static af_status dalloc(af_plan *p, void **dst, size_t bytes) {
 static void release(af_plan *p) {
   for (void *q : p->allocs) (void)hipFree(q);
   p->allocs.clear();
-  for (void **q : {&p->hx_in, &p->hx_out, &p->hx_masks})
+  for (void **q : {&p->hx_in, &p->hx_out, &p->hx_masks, &p->hx_img})
     if (*q) (void)hipFree(*q), *q = nullptr;
   for (int b = 0; b < 2; ++b) {
     if (p->stage[b]) (void)hipHostFree(p->stage[b]);
@@ -1118,8 +1118,10 @@ hipError_t download_staged(af_plan *p, void *h, const void *d, size_t n, hipStre
 
 }  // namespace
 
-af_status af_execute_host(af_plan *p, const void *h_in, void *h_out, uint8_t *h_masks) {
-  if (!p || !h_in || !h_out) return fail(AF_ERR_INVALID, "null argument");
+// Upload, execute, then download the plane (h_out) or, with h_images, the
+// DM-time images of slices of sps columns made on the device from the plane.
+static af_status host_call(af_plan *p, const void *h_in, void *h_out, uint8_t *h_masks,
+                           size_t sps, int isize, uint8_t *h_images) {
   AF_HIP(hipSetDevice(p->device));
   const size_t in_bytes = p->d.nsamp * (size_t)p->d.nchan * dtype_size(p->d.dtype);
   const size_t out_bytes = (size_t)p->ndm * p->nout * acc_size(p->d.dtype);
@@ -1131,6 +1133,12 @@ af_status af_execute_host(af_plan *p, const void *h_in, void *h_out, uint8_t *h_
   if (h_masks && m_bytes) {
     AF_TRY(grow(&p->hx_masks, &p->hx_masks_n, m_bytes));
     dm = (uint8_t *)p->hx_masks;
+  }
+  size_t img_bytes = 0;
+  if (h_images) {
+    const size_t nsl = (p->nout + sps - 1) / sps;
+    img_bytes = nsl * (size_t)isize * (size_t)isize * 3;
+    AF_TRY(grow(&p->hx_img, &p->hx_img_n, img_bytes));
   }
   hipStream_t st = p->stream;
   const bool staged = staging(p);
@@ -1145,8 +1153,15 @@ af_status af_execute_host(af_plan *p, const void *h_in, void *h_out, uint8_t *h_
     (void)hipStreamSynchronize(st);
     return s;
   }
-  e = staged ? download_staged(p, h_out, p->hx_out, out_bytes, st)
-             : hipMemcpyAsync(h_out, p->hx_out, out_bytes, hipMemcpyDeviceToHost, st);
+  if (h_images) {
+    const int pt = p->d.dtype == AF_F32 ? AF_F32 : p->d.dtype == AF_U32 ? AF_U64 : AF_U32;
+    AF_TRY(image_slices(p->hx_out, pt, (size_t)p->ndm, p->nout, p->nout, sps, isize,
+                        (uint8_t *)p->hx_img, st));
+    e = hipMemcpyAsync(h_images, p->hx_img, img_bytes, hipMemcpyDeviceToHost, st);
+  } else {
+    e = staged ? download_staged(p, h_out, p->hx_out, out_bytes, st)
+               : hipMemcpyAsync(h_out, p->hx_out, out_bytes, hipMemcpyDeviceToHost, st);
+  }
   if (e == hipSuccess && dm) e = hipMemcpyAsync(h_masks, dm, m_bytes, hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) {
@@ -1154,6 +1169,19 @@ af_status af_execute_host(af_plan *p, const void *h_in, void *h_out, uint8_t *h_
     return fail(AF_ERR_RUNTIME, std::string("D2H: ") + hipGetErrorString(e));
   }
   return AF_OK;
+}
+
+af_status af_execute_host(af_plan *p, const void *h_in, void *h_out, uint8_t *h_masks) {
+  if (!p || !h_in || !h_out) return fail(AF_ERR_INVALID, "null argument");
+  return host_call(p, h_in, h_out, h_masks, 0, 0, nullptr);
+}
+
+af_status af_execute_host_images(af_plan *p, const void *h_in, size_t sps, int size,
+                                 uint8_t *h_images, uint8_t *h_masks) {
+  if (!p || !h_in || !h_images) return fail(AF_ERR_INVALID, "null argument");
+  if (sps == 0 || size <= 0 || size > 8192)
+    return fail(AF_ERR_INVALID, "zero slice width or bad image size");
+  return host_call(p, h_in, nullptr, h_masks, sps, size, h_images);
 }
 
 af_status af_execute_noalloc(const af_plan_desc *desc, const void *d_in, void *d_out,

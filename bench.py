@@ -224,6 +224,13 @@ def served_leg(threads, chunks_per_beam, full_rfi=False):
     t0 = time.perf_counter()
     plan.run_host(xw)
     host_ms = (time.perf_counter() - t0) * 1e3
+    # the drop-in call of the search loop (_dedisperse_spec): host rows in,
+    # DM-time images of t_sample = one chunk out, the plane imaged on the
+    # device and never downloaded
+    plan.run_host_images(xw, b.chunk_rows)
+    t0 = time.perf_counter()
+    plan.run_host_images(xw, b.chunk_rows)
+    images_ms = (time.perf_counter() - t0) * 1e3
     d_in = capi.DeviceBuffer(xw.nbytes)
     d_in.upload(xw)
     d_out = capi.DeviceBuffer(out.nbytes)
@@ -249,6 +256,7 @@ def served_leg(threads, chunks_per_beam, full_rfi=False):
         "hbm_peak_GB": st["hbm_peak"] / 1e9,
         "h2d_pinned_GBps": h2d_gbs,
         "plain_single_call": {"chunk_s": b.chunk_s, "host_buffers_ms": host_ms,
+                              "host_buffers_images_ms": images_ms,
                               "device_resident_ms": dev_ms,
                               "host_buffers_rate": work / (host_ms / 1e3),
                               "device_resident_rate": work / (dev_ms / 1e3)},

@@ -40,6 +40,8 @@ typedef int af_status;
 #define AF_U16 2
 #define AF_U32 3
 #define AF_F32 4
+/* Element type of the plane of u32 samples (af_image_slices only). */
+#define AF_U64 5
 
 #define AF_METHOD_DIRECT 0
 #define AF_METHOD_SUBBAND 1
@@ -181,6 +183,30 @@ af_status af_execute_noalloc(const af_plan_desc *desc, const void *d_in, void *d
  * staging buffers; all are freed by af_plan_destroy. */
 af_status af_execute_host(af_plan *plan, const void *h_in, void *h_out,
                           uint8_t *h_masks);
+
+/* DM-time images of a device plane (replaces the preprocessing that turns
+ * the plane into DedispersedDataUint8 images, include/data.h:619-727, and
+ * its OpenCV calls cv::resize INTER_AREA / INTER_LANCZOS4, cv::normalize
+ * NORM_MINMAX, cv::applyColorMap COLORMAP_VIRIDIS):
+ *   d_plane : [rows][ld] plane elements, plane_dtype AF_U32 / AF_U64 /
+ *             AF_F32, columns [0, cols) used
+ *   sps     : columns per slice; nslices = ceil(cols / sps), the last one
+ *             cols - (nslices - 1) * sps wide
+ *   size    : image side (512 in the reference)
+ *   d_images: [nslices][size][size][3] bytes, BGR
+ * Each slice is resized to size x size (area averaging when both of its
+ * sides shrink, 8-tap Lanczos a = 4 otherwise), min-max normalised to 0..255
+ * (round half to even; a flat slice maps to 0) and mapped through viridis.
+ * Allocates its scratch and synchronises the stream before returning. */
+af_status af_image_slices(const void *d_plane, int plane_dtype, size_t rows, size_t cols,
+                          size_t ld, size_t sps, int size, uint8_t *d_images, void *stream);
+
+/* af_execute_host whose result is the DM-time images of the plane instead of
+ * the plane (the drop-in for _dedisperse_spec, src/bindpy/pyapi.cpp:286-309):
+ * the plane stays on the device, only h_images ([nslices][size][size][3],
+ * nslices = ceil(nout / sps)) and the optional masks are downloaded. */
+af_status af_execute_host_images(af_plan *plan, const void *h_in, size_t sps, int size,
+                                 uint8_t *h_images, uint8_t *h_masks);
 
 /* Properties of a device as the kernels see them. */
 typedef struct af_device_props {

@@ -88,3 +88,20 @@ def test_batch_rejects_the_same_plan_twice():
     assert rc == capi.AF_ERR_INVALID
     assert b"more than once" in lib.af_last_error()
     assert not out.value
+
+
+def test_image_slices_rejects_bad_arguments_before_any_device_call():
+    lib = capi.lib()
+    p = C.c_void_p(4096)   # never dereferenced: every case fails its argument checks first
+    cases = [
+        (p, 9, 10, 10, 10, 5, 512),                 # no such plane element type
+        (p, capi.AF_U32, 10, 10, 10, 0, 512),       # zero slice width
+        (p, capi.AF_U32, 0, 10, 10, 5, 512),        # no rows
+        (p, capi.AF_F32, 10, 20, 10, 5, 512),       # ld < cols
+        (p, capi.AF_U64, 10, 10, 10, 5, 0),         # image size 0
+    ]
+    for plane, dt, rows, cols, ld, sps, size in cases:
+        rc = lib.af_image_slices(plane, dt, rows, cols, ld, sps, size, p, None)
+        assert rc == capi.AF_ERR_INVALID, (dt, rows, cols, ld, sps, size)
+    assert lib.af_image_slices(None, capi.AF_F32, 1, 1, 1, 1, 8, p, None) == capi.AF_ERR_INVALID
+    assert lib.af_execute_host_images(None, p, 10, 512, p, None) == capi.AF_ERR_INVALID
