@@ -93,6 +93,21 @@ class ServerStats(C.Structure):
                 ("inflight_peak", C.c_int)]
 
 
+class SnrParams(C.Structure):
+    _fields_ = [("threshold_sigma", C.c_double), ("has_toa", C.c_int), ("toa", C.c_int),
+                ("has_window", C.c_int), ("window", C.c_int), ("has_tsamp", C.c_int),
+                ("has_target", C.c_int), ("tsamp", C.c_double), ("target_time_us", C.c_double),
+                ("has_noise_ranges", C.c_int), ("n_noise_ranges", C.c_int),
+                ("noise_ranges", C.c_void_p)]
+
+
+class SnrResult(C.Structure):
+    _fields_ = [("snr", C.c_double), ("width", C.c_int), ("peak", C.c_int),
+                ("noise_mean", C.c_double), ("noise_std", C.c_double), ("method", C.c_int),
+                ("converged", C.c_int), ("search_start", C.c_int), ("search_end", C.c_int),
+                ("downsample", C.c_int)]
+
+
 _lib = None
 
 # name -> (restype, argtypes); every symbol declared in include/astroflow_amd.h
@@ -141,6 +156,7 @@ SIGNATURES = {
     "af_server_release": (_I, [_P, _P]),
     "af_server_drain": (_I, [_P]),
     "af_server_get_stats": (_I, [_P, C.POINTER(ServerStats)]),
+    "af_frb_snr": (_I, [_P, _I, _I, _I, C.POINTER(SnrParams), C.POINTER(SnrResult), _P]),
     "af_last_error": (C.c_char_p, []),
     "af_version": (C.c_char_p, []),
 }
@@ -542,3 +558,40 @@ def autotune(spec: "DedispSpec", d_in: int, d_out: int, tune_subbands: bool = Fa
     tuned = dataclasses.replace(spec, tile_t=desc.tile_t, dm_per_block=desc.dm_per_block,
                                 sbw=desc.sb_nchan)
     return tuned, ms.value
+
+
+def frb_snr(d_spec: int, dtype: int, ntime: int, nfreq: int, noise_range=None,
+            threshold_sigma: float = 5.0, toa_sample_idx=None, fitting_window_samples=None,
+            tsamp=None, target_time_us=None, stream: int = 0):
+    """af_frb_snr on a device spectrum, returned as _calculate_frb_snr does:
+    (snr, width, peak, (noise_mean, noise_std, info))."""
+    p = SnrParams()
+    p.threshold_sigma = threshold_sigma
+    pairs = []
+    if noise_range is not None and len(noise_range) > 0:
+        p.has_noise_ranges = 1
+        pairs = [(int(r[0]), int(r[1])) for r in noise_range if len(r) == 2]
+    arr = np.asarray(pairs, np.int32).reshape(-1, 2) if pairs else np.zeros((0, 2), np.int32)
+    p.n_noise_ranges = len(pairs)
+    p.noise_ranges = arr.ctypes.data if pairs else None
+    if toa_sample_idx is not None:
+        p.has_toa, p.toa = 1, int(toa_sample_idx)
+    if fitting_window_samples is not None:
+        p.has_window, p.window = 1, int(fitting_window_samples)
+    if tsamp is not None:
+        p.has_tsamp, p.tsamp = 1, float(tsamp)
+    if target_time_us is not None:
+        p.has_target, p.target_time_us = 1, float(target_time_us)
+    r = SnrResult()
+    check(lib().af_frb_snr(C.c_void_p(d_spec) if d_spec else None, dtype, ntime, nfreq,
+                           C.byref(p), C.byref(r), C.c_void_p(stream) if stream else None),
+          "af_frb_snr")
+    info = {"fit_converged": bool(r.converged)}
+    if r.method == 0:
+        return (-1.0, 0, 0, (0.0, 1.0, info))
+    if r.method == 2:
+        info["method"] = "fallback"
+        return (r.snr, r.width, r.peak, (r.noise_mean, r.noise_std, info))
+    info.update({"method": "boxcar", "width_samples": r.width, "search_start": r.search_start,
+                 "search_end": r.search_end, "downsample": r.downsample})
+    return (r.snr, r.width, r.peak, (r.noise_mean, r.noise_std, info))

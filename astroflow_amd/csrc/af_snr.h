@@ -7,8 +7,10 @@
 // boxcar of widths 1..16 plus 32 log-spaced widths up to the fitting window
 // is slid over the series (within a search radius around a given TOA).  The
 // best window's noise is then re-estimated without the samples near it.
-// The reference runs this on the host after a candidate is found; it is
-// small (one time series) and stays on the host here too.
+// The reference runs this on the host after a candidate is found; so does
+// the drop-in `_calculate_frb_snr`.  af_snr.hip runs the same steps for a
+// spectrum already on the device (the per-sample sums and the boxcar search
+// there, the order statistics here).
 #pragma once
 
 #include <algorithm>
@@ -129,20 +131,26 @@ struct SnrResult {
   int search_start = 0, search_end = 0, downsample = 1;
 };
 
-inline SnrResult boxcar_snr(const SnrInput &in) {
-  SnrResult res;
-  if (in.ntime <= 0 || in.nfreq <= 0) {
-    res.empty = true;
-    return res;
-  }
-  // frequency-summed series, in row order
-  std::vector<double> ts((size_t)in.ntime);
-  for (int t = 0; t < in.ntime; ++t) {
-    const float *row = in.spec + (size_t)t * (size_t)in.nfreq;
-    double s = 0.0;
-    for (int f = 0; f < in.nfreq; ++f) s += row[f];
-    ts[(size_t)t] = s;
-  }
+// The series after the frequency sums, decimated, with everything the
+// boxcar search needs: noise statistics, prefix sums, widths, search range.
+struct SnrPrep {
+  std::vector<double> ts, cs;
+  std::vector<int> widths;   // ascending; those >= n are not searched
+  int ds = 1, n = 0, s0 = 0, s1 = 0, argmax = 0;
+  double mu = 0.0, sd = 1.0;
+};
+
+// Best boxcar of one width: the first start (lowest index) with the largest
+// S/N among centres inside the search range.
+struct WidthBest {
+  bool any = false;
+  double snr = 0.0, sum = 0.0;
+  int centre = 0;
+};
+
+// ts: the frequency-summed series at the native rate (ntime values)
+inline SnrPrep snr_prepare(const SnrInput &in, std::vector<double> ts) {
+  SnrPrep p;
   // decimation to the target resolution (whole groups only)
   int ds = 1;
   if (in.has_tsamp && in.has_target) {
@@ -194,73 +202,120 @@ inline SnrResult boxcar_snr(const SnrInput &in) {
     }
     if (noise.size() < 10) noise = ts;
   }
-  auto [mu, sd] = clipped_stats(noise, in.sigma);
+  const auto st = clipped_stats(noise, in.sigma);
+  p.mu = st.first;
+  p.sd = st.second;
+  p.cs.assign((size_t)n + 1, 0.0);
+  for (int i = 0; i < n; ++i) p.cs[(size_t)i + 1] = p.cs[(size_t)i] + ts[(size_t)i];
+  p.argmax = (int)(std::max_element(ts.begin(), ts.end()) - ts.begin());
+  p.widths = boxcar_widths(wmax_ds);
+  p.ds = ds;
+  p.n = n;
+  p.s0 = s0;
+  p.s1 = s1;
+  p.ts.swap(ts);
+  return p;
+}
 
-  std::vector<double> cs((size_t)n + 1, 0.0);
-  for (int i = 0; i < n; ++i) cs[(size_t)i + 1] = cs[(size_t)i] + ts[(size_t)i];
-  const int argmax = (int)(std::max_element(ts.begin(), ts.end()) - ts.begin());
-
+// the boxcar of width w at every start, on the host
+inline WidthBest search_width(const SnrInput &in, const SnrPrep &p, int w) {
   const double ninf = -std::numeric_limits<double>::infinity();
-  double best = ninf, best_sum = std::numeric_limits<double>::quiet_NaN();
-  int best_w = 1, best_c = argmax;
-  for (int w : boxcar_widths(wmax_ds)) {
-    if (w >= n) break;
-    bool any = false;
-    double lb = ninf, lsum = 0.0;
-    int lc = best_c;
-    for (int i = 0; i + w <= n; ++i) {
-      const int c = i + w / 2;
-      if (in.has_toa && (c < s0 || c >= s1)) continue;
-      const double sum = cs[(size_t)i + w] - cs[(size_t)i];
-      const double snr = sd > 0.0 ? (sum - mu * w) / (sd * std::sqrt((double)w)) : ninf;
-      if (!any || snr > lb) {
-        any = true;
-        lb = snr;
-        lc = c;
-        lsum = sum;
-      }
-    }
-    if (any && lb > best) {
-      best = lb;
-      best_w = w;
-      best_c = lc;
-      best_sum = lsum;
+  WidthBest b;
+  b.snr = ninf;
+  for (int i = 0; i + w <= p.n; ++i) {
+    const int c = i + w / 2;
+    if (in.has_toa && (c < p.s0 || c >= p.s1)) continue;
+    const double sum = p.cs[(size_t)i + w] - p.cs[(size_t)i];
+    const double snr = p.sd > 0.0 ? (sum - p.mu * w) / (p.sd * std::sqrt((double)w)) : ninf;
+    if (!b.any || snr > b.snr) {
+      b.any = true;
+      b.snr = snr;
+      b.centre = c;
+      b.sum = sum;
     }
   }
+  return b;
+}
 
+// best[k]: the search at widths[k] (for the widths below n, in order)
+inline SnrResult snr_finish(const SnrInput &in, SnrPrep &p, const std::vector<WidthBest> &best_w) {
+  SnrResult res;
+  const int n = p.n, ds = p.ds;
+  const double ninf = -std::numeric_limits<double>::infinity();
+  double best = ninf, best_sum = std::numeric_limits<double>::quiet_NaN();
+  int bw = 1, best_c = p.argmax;
+  for (size_t k = 0; k < p.widths.size() && k < best_w.size(); ++k) {
+    const int w = p.widths[k];
+    if (w >= n) break;
+    const WidthBest &b = best_w[k];
+    if (b.any && b.snr > best) {
+      best = b.snr;
+      bw = w;
+      best_c = b.centre;
+      best_sum = b.sum;
+    }
+  }
   res.downsample = ds;
   if (!std::isfinite(best_sum)) {
     // no usable boxcar: peak sample against the whole series' clipped stats
-    auto [m2, s2] = clipped_stats(ts, in.sigma);
+    auto [m2, s2] = clipped_stats(p.ts, in.sigma);
     res.noise_mean = m2;
     res.noise_std = s2;
-    res.snr = s2 > 0.0 ? (ts[(size_t)argmax] - m2) / s2 : -1.0;
+    res.snr = s2 > 0.0 ? (p.ts[(size_t)p.argmax] - m2) / s2 : -1.0;
     res.width = std::max(1, ds);
-    res.peak = std::min(in.ntime - 1, argmax * ds + ds / 2);
+    res.peak = std::min(in.ntime - 1, p.argmax * ds + ds / 2);
     res.fallback = true;
     return res;
   }
   // noise again, without the samples within two widths of the best window
-  const int rl = std::max(0, best_c - 2 * best_w);
-  const int rr = std::min(n, best_c + 2 * best_w + 1);
+  double mu = p.mu, sd = p.sd;
+  const int rl = std::max(0, best_c - 2 * bw);
+  const int rr = std::min(n, best_c + 2 * bw + 1);
   std::vector<double> rest;
   rest.reserve((size_t)n);
   for (int i = 0; i < n; ++i)
-    if (i < rl || i >= rr) rest.push_back(ts[(size_t)i]);
+    if (i < rl || i >= rr) rest.push_back(p.ts[(size_t)i]);
   if (rest.size() >= std::max<size_t>(10, (size_t)(0.1 * (double)n))) {
     auto st = clipped_stats(rest, in.sigma);
     mu = st.first;
     sd = st.second;
   }
-  res.snr = sd > 0.0 ? (best_sum - mu * best_w) / (sd * std::sqrt((double)best_w)) : -1.0;
+  res.snr = sd > 0.0 ? (best_sum - mu * bw) / (sd * std::sqrt((double)bw)) : -1.0;
   res.noise_mean = mu;
   res.noise_std = sd;
-  res.width = std::max(1, best_w * ds);
+  res.width = std::max(1, bw * ds);
   res.peak = std::min(in.ntime - 1, best_c * ds + ds / 2);
   res.converged = true;
-  res.search_start = s0;
-  res.search_end = s1;
+  res.search_start = p.s0;
+  res.search_end = p.s1;
   return res;
+}
+
+// frequency-summed series, in row order, in double
+inline std::vector<double> freq_sums(const SnrInput &in) {
+  std::vector<double> ts((size_t)in.ntime);
+  for (int t = 0; t < in.ntime; ++t) {
+    const float *row = in.spec + (size_t)t * (size_t)in.nfreq;
+    double s = 0.0;
+    for (int f = 0; f < in.nfreq; ++f) s += row[f];
+    ts[(size_t)t] = s;
+  }
+  return ts;
+}
+
+inline SnrResult boxcar_snr(const SnrInput &in) {
+  if (in.ntime <= 0 || in.nfreq <= 0) {
+    SnrResult res;
+    res.empty = true;
+    return res;
+  }
+  SnrPrep p = snr_prepare(in, freq_sums(in));
+  std::vector<WidthBest> bw;
+  for (int w : p.widths) {
+    if (w >= p.n) break;
+    bw.push_back(search_width(in, p, w));
+  }
+  return snr_finish(in, p, bw);
 }
 
 }  // namespace afsnr

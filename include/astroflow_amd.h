@@ -158,6 +158,41 @@ af_status af_execute(af_plan *plan, const void *d_in, void *d_out,
 af_status af_execute_spectrum(af_plan *plan, const void *d_in, void *d_out, size_t t0,
                               size_t t_len, uint8_t *d_masks, void *stream);
 
+/* Boxcar S/N of a dynamic spectrum on the device (replaces the host function
+ * behind _calculate_frb_snr, src/bindpy/frb_snr.cpp:144-398, for a spectrum
+ * already in HBM such as af_execute_spectrum's output).  d_spec is
+ * [ntime][nfreq] of dtype (AF_U8 .. AF_F32; values are taken as float32, as
+ * the reference's float array).  The rows are summed over frequency in
+ * double, decimated to target_time_us when tsamp and the target are given,
+ * the noise is a sigma-clipped median/MAD estimate (over noise_ranges, or
+ * off the TOA window, or the whole series), boxcars of widths 1..16 and up
+ * to 32 log-spaced wider ones are slid over the series (centres within the
+ * search radius of the TOA when given), and the best window's noise is
+ * re-estimated without it.  Synchronises the stream. */
+typedef struct af_snr_params {
+  double threshold_sigma;  /* clipping threshold (5.0)                          */
+  int has_toa, toa;        /* TOA (native-rate sample index)                    */
+  int has_window, window;  /* fitting window (native-rate samples)              */
+  int has_tsamp, has_target;
+  double tsamp;            /* seconds                                           */
+  double target_time_us;   /* decimate to this resolution                       */
+  int has_noise_ranges;    /* a list was given (even with no usable pair)       */
+  int n_noise_ranges;
+  const int *noise_ranges; /* host [n][2]: native-rate [start, end)             */
+} af_snr_params;
+
+typedef struct af_snr_result {
+  double snr;              /* -1 when there is no data                          */
+  int width, peak;         /* native-rate samples                               */
+  double noise_mean, noise_std;
+  int method;              /* 0 no data, 1 boxcar, 2 fallback (peak sample)     */
+  int converged;
+  int search_start, search_end, downsample;
+} af_snr_result;
+
+af_status af_frb_snr(const void *d_spec, int dtype, int ntime, int nfreq,
+                     const af_snr_params *params, af_snr_result *result, void *stream);
+
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory: code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc.  af_plan_create + af_execute + af_plan_destroy in one

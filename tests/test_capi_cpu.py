@@ -55,7 +55,9 @@ def _c_sizeof(type_name, tmp_path):
                                           ("af_device_props", capi.DeviceProps),
                                           ("af_server_desc", capi.ServerDesc),
                                           ("af_server_result", capi.ServerResult),
-                                          ("af_server_stats", capi.ServerStats)])
+                                          ("af_server_stats", capi.ServerStats),
+                                          ("af_snr_params", capi.SnrParams),
+                                          ("af_snr_result", capi.SnrResult)])
 def test_struct_layouts_match_c(cname, pytype, tmp_path):
     assert _c_sizeof(cname, tmp_path) == C.sizeof(pytype)
 
@@ -105,3 +107,10 @@ def test_image_slices_rejects_bad_arguments_before_any_device_call():
         assert rc == capi.AF_ERR_INVALID, (dt, rows, cols, ld, sps, size)
     assert lib.af_image_slices(None, capi.AF_F32, 1, 1, 1, 1, 8, p, None) == capi.AF_ERR_INVALID
     assert lib.af_execute_host_images(None, p, 10, 512, p, None) == capi.AF_ERR_INVALID
+
+
+def test_frb_snr_without_data_returns_no_method():
+    # an empty spectrum answers before any device call
+    assert capi.frb_snr(0, capi.AF_F32, 0, 4) == (-1.0, 0, 0, (0.0, 1.0, {"fit_converged": False}))
+    with pytest.raises(ValueError):
+        capi.frb_snr(0, 9, 10, 4)
