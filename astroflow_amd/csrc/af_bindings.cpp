@@ -22,6 +22,7 @@
 #include <string>
 #include <vector>
 
+#include "af_cpu.h"
 #include "af_image.h"
 #include "af_snr.h"
 #include "astroflow_amd.h"
@@ -674,14 +675,99 @@ struct Filterbank {
   }
 };
 
+// target 0 of _dedispered_fil: the host cores (include/cpucal.hpp:75-201,
+// checks and slicing as there; delays, layout and imaging in af_cpu.h /
+// af_image.h).  Every slice is imaged at its own width (the reference
+// images the last, shorter slice at the first slice's width, reading past
+// its buffer).
+template <typename T>
+DedispersedDataUint8 dedispered_fil_cpu(const Filterbank &fil, float dm_low, float dm_high,
+                                        float freq_start, float freq_end, float dm_step, int ds,
+                                        float t_sample) {
+  const Header h = fil.header();
+  const size_t nchans = (size_t)h.nchans, ndata = (size_t)h.ndata;
+  if (nchans == 0 || ndata == 0) throw py::value_error("empty filterbank");
+  std::vector<double> freq(nchans);
+  for (size_t i = 0; i < nchans; ++i) freq[i] = h.fch1 + (float)i * h.foff;   // float fields
+  const float fmin = (float)freq.front(), fmax = (float)freq.back();
+  if (freq_start < fmin || freq_end > fmax) {
+    char msg[256];
+    std::snprintf(msg, sizeof(msg),
+                  "Frequency range [%.3f-%.3f MHz] out of filterbank range [%.3f-%.3f MHz]",
+                  freq_start, freq_end, fmin, fmax);
+    throw py::value_error(msg);
+  }
+  const float span = fmax - fmin;
+  size_t cs = (size_t)((freq_start - fmin) / span * (float)(nchans - 1));
+  size_t ce = (size_t)((freq_end - fmin) / span * (float)(nchans - 1));
+  ce = std::min(nchans - 1, ce);
+  if (cs >= nchans || ce >= nchans || cs >= ce) throw py::value_error("Invalid chan parameters");
+  if (ds < 1) throw py::value_error("time_downsample must be >= 1");
+  if (dm_low > dm_high || dm_low < 0 || !(dm_step > 0.0f))
+    throw py::value_error("Invalid DM parameters");
+  if ((double)t_sample > (double)ndata * fil.tsamp)
+    throw py::value_error("t_sample exceeds total observation time");
+  const size_t sps = (size_t)std::round((double)t_sample / fil.tsamp);
+  if (sps == 0) throw py::value_error("t_sample is shorter than one sample");
+  const size_t ndm = (size_t)((dm_high - dm_low) / dm_step) + 1;
+  const size_t nslices = (ndata + sps - 1) / sps;
+  constexpr int kSize = 512;
+  const size_t img_bytes = (size_t)kSize * kSize * 3;
+  std::vector<uint8_t> buf(nslices * img_bytes);
+  {
+    py::gil_scoped_release nogil;
+    const std::vector<int32_t> delays =
+        afcpu::delay_table(freq, cs, ce, dm_low, dm_step, ndm, (double)h.tsamp);
+    const std::vector<T> xt = afcpu::channel_major((const T *)fil.raw->data(), ndata, nchans, cs, ce);
+    std::vector<uint64_t> plane;
+    for (size_t s = 0; s < nslices; ++s) {
+      const size_t start = s * sps, len = std::min(sps, ndata - start);
+      const size_t down = (len + (size_t)ds - 1) / (size_t)ds;
+      plane.resize(ndm * down);
+      afcpu::dedisperse_slice(xt, ndata, ce - cs, delays, ndm, start, len, ds, plane.data());
+      afimg::slice_image(plane.data(), down, 0, (int)ndm, (int)down, kSize,
+                         buf.data() + s * img_bytes);
+    }
+  }
+  py::list images;
+  for (size_t s = 0; s < nslices; ++s) {
+    py::array_t<uint8_t> a({kSize, kSize, 3});
+    std::memcpy(a.mutable_data(), buf.data() + s * img_bytes, img_bytes);
+    images.append(a);
+  }
+  DedispersedDataUint8 out;
+  out.dm_times = images;
+  out.shape = {512, 512, 3};
+  out.dm_ndata = 512;
+  out.downtsample_ndata = 512;
+  out.dm_low = dm_low;
+  out.dm_high = dm_high;
+  out.dm_step = dm_step;
+  out.tsample = t_sample;
+  out.filname = h.filename;
+  return out;
+}
+
 DedispersedDataUint8 dedispered_fil(std::string filename, float dm_low, float dm_high,
                                     float freq_start, float freq_end, float dm_step,
                                     int time_downsample, float t_sample, int target, int target_id,
                                     std::string mask_file, RFIConfig rficfg) {
-  if (target != 1)
-    throw std::runtime_error("target 0 (the CPU dedispersion) is not part of this library; "
-                             "use target=1");
+  if (target != 0 && target != 1)
+    throw py::value_error("target must be 0 (host cores) or 1 (GPU)");
   Filterbank fil(filename);
+  if (target == 0) {
+    switch (fil.nbits) {
+    case 8:
+      return dedispered_fil_cpu<uint8_t>(fil, dm_low, dm_high, freq_start, freq_end, dm_step,
+                                         time_downsample, t_sample);
+    case 16:
+      return dedispered_fil_cpu<uint16_t>(fil, dm_low, dm_high, freq_start, freq_end, dm_step,
+                                          time_downsample, t_sample);
+    default:
+      return dedispered_fil_cpu<uint32_t>(fil, dm_low, dm_high, freq_start, freq_end, dm_step,
+                                          time_downsample, t_sample);
+    }
+  }
   const Header h = fil.header();
   py::array a = fil.data();
   switch (fil.nbits) {
