@@ -556,7 +556,7 @@ def autotune(spec: "DedispSpec", d_in: int, d_out: int, tune_subbands: bool = Fa
     check(lib().af_autotune(C.byref(desc), C.c_void_p(d_in), C.c_void_p(d_out),
                             int(tune_subbands), reps, C.byref(ms)), "af_autotune")
     tuned = dataclasses.replace(spec, tile_t=desc.tile_t, dm_per_block=desc.dm_per_block,
-                                sbw=desc.sb_nchan)
+                                sbw=desc.sb_nchan, ndm0=desc.ndm0)
     return tuned, ms.value
 
 

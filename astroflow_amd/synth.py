@@ -227,18 +227,51 @@ def noise_f32_threaded(seed: int, nsamp: int, nchan: int, threads: int = 8,
     return out
 
 
+def _normal_lut(dtype) -> np.ndarray:
+    """2^16 equal-probability levels of N(0,1) (the midpoint quantiles), as
+    float32, or as uint8 N(128,16) rounded and clipped to 0..255."""
+    from scipy.special import ndtri
+    q = ndtri((np.arange(65536, dtype=np.float64) + 0.5) / 65536.0)
+    if dtype == np.uint8:
+        return np.clip(np.rint(128.0 + 16.0 * q), 0, 255).astype(np.uint8)
+    return q.astype(np.float32)
+
+
+def noise_lut_threaded(seed: int, nsamp: int, nchan: int, dtype, threads: int = 8,
+                       rows_per_block: int = 8192) -> np.ndarray:
+    """Normal noise by table look-up of uniform 16-bit draws (float32 N(0,1),
+    or uint8 N(128,16)): several times faster than drawing normals, in fixed
+    blocks of rows from spawned seeds (independent of the thread count)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    lut = _normal_lut(dtype)
+    out = np.empty((nsamp, nchan), dtype)
+    nblk = (nsamp + rows_per_block - 1) // rows_per_block
+    seeds = np.random.SeedSequence(seed).spawn(nblk)
+
+    def work(b):
+        r0 = b * rows_per_block
+        r1 = min(nsamp, r0 + rows_per_block)
+        u = np.random.default_rng(seeds[b]).integers(0, 65536, size=(r1 - r0, nchan),
+                                                     dtype=np.uint16)
+        np.take(lut, u, out=out[r0:r1])
+
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        list(ex.map(work, range(nblk)))
+    return out
+
+
 def c3_chunk_data(c: dict, seed: int, threads: int = 8) -> np.ndarray:
     """Seeded input of one batched beam-chunk (a dict from c3_chunks): uint8
-    N(128,16) or float32 N(0,1) noise, plus narrowband RFI on 5% of the
+    N(128,16) or float32 N(0,1) noise (noise_lut_threaded), plus narrowband RFI on 5% of the
     channels (sinusoids of amplitude 40 for uint8 and 2.5 sigma for float32,
     periods log-uniform in 8-4096 samples) and 3 channels stuck at the
     maximum (255 / 8.0), so that IQRM has channels to flag."""
     nsamp, nchan = c["nsamp"], c["nchan"]
+    x = noise_lut_threaded(seed, nsamp, nchan, c["dtype"], threads=threads)
     if c["dtype"] == np.uint8:
-        x = noise_u8_threaded(seed, nsamp, nchan, threads=threads)
         amp, top = 40.0, 255.0
     else:
-        x = noise_f32_threaded(seed, nsamp, nchan, threads=threads)
         amp, top = 2.5, 8.0
     rng = np.random.default_rng(seed + 7919)
     k = max(1, int(round(0.05 * nchan)))

@@ -287,33 +287,47 @@ def c3_spec(c):
         use_zero_dm=True, zdm_c0=0, zdm_c1=nchan - 1)
 
 
-def batch_leg(threads, budget_gb=110.0):
+def batch_leg(threads, budget_gb=40.0, reps=5):
     """The 48 seeded beam-chunks (uneven nchan, nsamp, DM range, dtype, ds),
-    each with IQRM + zero-DM: one plain execute per chunk (the reference's
-    per-call pattern, inputs resident) vs the batched launches (one launch
-    per stage and kernel) vs their hipGraph replay.  Chunks are grouped under
-    an HBM budget; every mode runs each group once after a warm-up."""
+    each with IQRM + zero-DM and its own seeded input (synth.c3_chunk_data,
+    seed 1000 + i, the inputs of the batched parity test), resident in HBM.
+    Chunks are grouped under an HBM budget.  Per group, each mode runs after
+    a warm-up `reps` times and its median counts:
+      plain        one execute per chunk, default settings (the reference's
+                   per-call pattern);
+      plain_tuned  the same with each chunk's settings from af_autotune
+                   (stage-1 form and stage-2 tile; the plane is unchanged);
+      merged       the tuned plans as one af_batch: one launch per stage and
+                   kernel for the whole group;
+      graph        that batch replayed from its hipGraph.
+    device_ms is the sum of the chunks' per-stage HIP-event times (the GPU's
+    busy time); wall time above it is what merging launches can remove."""
     from astroflow_amd import capi, synth
     chunks = synth.c3_chunks(seed=4)
-    pool_rows = 1 << 16
-    rng = np.random.default_rng(40)
-    pools = {}
-    for dt in (np.uint8, np.float32):
-        n = pool_rows * 4096
-        if dt == np.uint8:
-            a = synth.noise_u8_threaded(41, pool_rows, 4096, threads=threads)
-        else:
-            a = rng.standard_normal(n, dtype=np.float32).reshape(pool_rows, 4096)
-        d = capi.DeviceBuffer(a.nbytes)
-        d.upload(a)
-        pools[dt] = (d, a.nbytes)
     budget = budget_gb * 1e9
-    t_plain = t_merged = t_graph = 0.0
+    modes = ("plain", "plain_tuned", "merged", "graph")
+    tot = {m: 0.0 for m in modes}
+    spread = {m: [] for m in modes}
+    device_s = 0.0
     work = 0
     launches = recorded = 0
     ngroups = 0
+    tuned_settings = []
+    t_gen = 0.0
     i = 0
     t_setup = time.perf_counter()
+
+    def timed(fn):
+        fn()
+        capi.sync()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            capi.sync()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), ts
+
     while i < len(chunks):
         group = []
         used = 0.0
@@ -326,74 +340,88 @@ def batch_leg(threads, budget_gb=110.0):
             if group and used + need > budget:
                 plan.close()
                 break
-            group.append((spec, plan))
+            group.append((i, spec, plan))
             used += need
             i += 1
         ngroups += 1
-        ins, outs = [], []
-        for spec, plan in group:
-            es = np.dtype(spec.dtype).itemsize
-            nb = spec.nsamp * spec.nchan * es
-            di = capi.DeviceBuffer(nb)
-            pool, pb = pools[spec.dtype]
-            off = 0
-            while off < nb:                      # fill from the noise pool
-                k = min(pb, nb - off)
-                capi.check(capi.lib().af_memcpy_d2d(capi.C.c_void_p(di.ptr + off),
-                                                    capi.C.c_void_p(pool.ptr), k, None))
-                off += k
+        ins, outs, tplans = [], [], []
+        for k, spec, plan in group:
+            t0 = time.perf_counter()
+            x = synth.c3_chunk_data(chunks[k], seed=1000 + k, threads=threads)
+            t_gen += time.perf_counter() - t0
+            di = capi.DeviceBuffer(x.nbytes)
+            di.upload(x)
+            del x
             do = capi.DeviceBuffer(plan.info.ndm * plan.info.nout * plan.info.out_elem_bytes)
             ins.append(di)
             outs.append(do)
             work += plan.info.ndm * plan.info.nout * spec.ds
+            tspec, _ = capi.autotune(spec, di.ptr, do.ptr, tune_subbands=False, reps=2)
+            tuned_settings.append((tspec.dm_per_block, tspec.tile_t))
+            tplans.append(capi.Plan(tspec))
         capi.sync()
-        plans = [p for _, p in group]
-        for p, di, do in zip(plans, ins, outs):            # warm-up
-            p.execute(di.ptr, do.ptr)
-        capi.sync()
-        t0 = time.perf_counter()
-        for p, di, do in zip(plans, ins, outs):
-            p.execute(di.ptr, do.ptr)
-        capi.sync()
-        t_plain += time.perf_counter() - t0
-        batch = capi.Batch(plans, [b.ptr for b in ins], [b.ptr for b in outs])
+        plans = [p for _, _, p in group]
+
+        def run(ps):
+            def f():
+                for p, di, do in zip(ps, ins, outs):
+                    p.execute(di.ptr, do.ptr)
+            return f
+        med = {}
+        med["plain"], ts = timed(run(plans))
+        spread["plain"] += ts
+        med["plain_tuned"], ts = timed(run(tplans))
+        spread["plain_tuned"] += ts
+        batch = capi.Batch(tplans, [b.ptr for b in ins], [b.ptr for b in outs])
         inf = batch.info()
         launches += inf["launches"]
         recorded += inf["recorded"]
-        batch.execute(use_graph=False)
-        capi.sync()
-        t0 = time.perf_counter()
-        batch.execute(use_graph=False)
-        capi.sync()
-        t_merged += time.perf_counter() - t0
-        batch.execute(use_graph=True)                     # capture + first replay
-        capi.sync()
-        t0 = time.perf_counter()
-        batch.execute(use_graph=True)
-        capi.sync()
-        t_graph += time.perf_counter() - t0
+        med["merged"], ts = timed(lambda: batch.execute(use_graph=False))
+        spread["merged"] += ts
+        med["graph"], ts = timed(lambda: batch.execute(use_graph=True))
+        spread["graph"] += ts
         batch.close()
-        for p in plans:
+        # GPU busy time of the tuned plans, from their stage events
+        for p, di, do in zip(tplans, ins, outs):
+            p.set_profiling(True)
+            for _ in range(2):
+                p.execute(di.ptr, do.ptr)
+            capi.sync()
+            device_s += p.stage_times()["total"] / 1e3
+            p.set_profiling(False)
+        for m in modes:
+            tot[m] += med[m]
+        for p in plans + tplans:
             p.close()
         for b in ins + outs:
             b.free()
-        log(f"batch group {ngroups}: {len(group)} chunks, {used / 1e9:.1f} GB; "
-            f"plain {t_plain:.3f} s merged {t_merged:.3f} s graph {t_graph:.3f} s so far")
-    for d, _ in pools.values():
-        d.free()
-    return {
+        log(f"batch group {ngroups}: {len(group)} chunks, {used / 1e9:.1f} GB; medians "
+            + " ".join(f"{m} {med[m] * 1e3:.1f} ms" for m in modes))
+    res = {
         "workload": "48 seeded beam-chunks: nchan 1024/2048/4096, nsamp log-uniform 2^14-2^20, "
-                    "u8 or f32 (N(0,1)), DM 0-500/2000/5000 step 1 or 0.5, ds 1/2/4; IQRM + "
-                    "zero-DM; 32-channel subbands; full-length planes; inputs resident in HBM",
-        "chunks": len(chunks), "groups": ngroups, "hbm_budget_GB": budget_gb,
+                    "u8 or f32, DM 0-500/2000/5000 step 1 or 0.5, ds 1/2/4; IQRM + zero-DM; "
+                    "32-channel subbands; full-length planes; each chunk its own seeded input "
+                    "(noise + narrowband RFI), resident in HBM",
+        "chunks": len(chunks), "groups": ngroups, "hbm_budget_GB": budget_gb, "reps": reps,
         "work": work, "launches_batched": launches, "launches_plain": recorded,
-        "plain_s": t_plain, "merged_s": t_merged, "graph_s": t_graph,
-        "plain_rate": work / t_plain, "merged_rate": work / t_merged,
-        "graph_rate": work / t_graph,
-        "speedup_merged_vs_plain": t_plain / t_merged,
-        "speedup_graph_vs_plain": t_plain / t_graph,
-        "setup_s": time.perf_counter() - t_setup - t_plain - t_merged - t_graph,
+        "tuned_settings": {f"dm_per_block {a} tile {b}": tuned_settings.count((a, b))
+                           for a, b in sorted(set(tuned_settings))},
+        "device_s": device_s,
+        "input_generation_s": t_gen,
+        "setup_s": time.perf_counter() - t_setup - sum(sum(v) for v in spread.values()),
     }
+    for m in modes:
+        res[f"{m}_s"] = tot[m]
+        res[f"{m}_rate"] = work / tot[m]
+        res[f"{m}_rep_s_min_max"] = [min(spread[m]), max(spread[m])]
+    res["speedup_tuned_vs_plain"] = tot["plain"] / tot["plain_tuned"]
+    res["speedup_merged_vs_plain"] = tot["plain"] / tot["merged"]
+    res["speedup_graph_vs_plain"] = tot["plain"] / tot["graph"]
+    res["speedup_graph_vs_plain_tuned"] = tot["plain_tuned"] / tot["graph"]
+    # the most any launch merging could give: plain-tuned wall time over GPU
+    # busy time
+    res["ceiling_launch_merging"] = tot["plain_tuned"] / device_s if device_s > 0 else None
+    return res
 
 
 def direct_leg(threads, cpu_target_s, reps=10, with_cpu=True):
