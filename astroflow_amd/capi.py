@@ -93,6 +93,13 @@ class ServerStats(C.Structure):
                 ("inflight_peak", C.c_int)]
 
 
+class FilInfo(C.Structure):
+    _fields_ = [("nchans", C.c_int), ("nbits", C.c_int), ("dtype", C.c_int), ("npol", C.c_int),
+                ("flipped", C.c_int), ("ndata", C.c_uint64), ("data_offset", C.c_uint64),
+                ("fch1", C.c_double), ("foff", C.c_double), ("tsamp", C.c_double),
+                ("tstart", C.c_double)]
+
+
 class SnrParams(C.Structure):
     _fields_ = [("threshold_sigma", C.c_double), ("has_toa", C.c_int), ("toa", C.c_int),
                 ("has_window", C.c_int), ("window", C.c_int), ("has_tsamp", C.c_int),
@@ -156,6 +163,12 @@ SIGNATURES = {
     "af_server_release": (_I, [_P, _P]),
     "af_server_drain": (_I, [_P]),
     "af_server_get_stats": (_I, [_P, C.POINTER(ServerStats)]),
+    "af_fil_open": (_I, [C.c_char_p, C.POINTER(_P), C.POINTER(FilInfo)]),
+    "af_fil_read": (_I, [_P, C.c_uint64, _S, _P]),
+    "af_fil_close": (_I, [_P]),
+    "af_psrfits_open": (_I, [C.c_char_p, C.POINTER(_P), C.POINTER(FilInfo)]),
+    "af_psrfits_read": (_I, [_P, C.c_uint64, _S, _P]),
+    "af_psrfits_close": (_I, [_P]),
     "af_frb_snr": (_I, [_P, _I, _I, _I, C.POINTER(SnrParams), C.POINTER(SnrResult), _P]),
     "af_last_error": (C.c_char_p, []),
     "af_version": (C.c_char_p, []),
@@ -595,3 +608,54 @@ def frb_snr(d_spec: int, dtype: int, ntime: int, nfreq: int, noise_range=None,
     info.update({"method": "boxcar", "width_samples": r.width, "search_start": r.search_start,
                  "search_end": r.search_end, "downsample": r.downsample})
     return (r.snr, r.width, r.peak, (r.noise_mean, r.noise_std, info))
+
+
+_DT = {AF_U8: np.uint8, AF_U16: np.uint16, AF_U32: np.uint32, AF_F32: np.float32}
+
+
+class FileReader:
+    """A SIGPROC (kind "fil") or PSRFITS (kind "psrfits") file opened through
+    the C-ABI readers: rows time-major, channels ascending."""
+
+    def __init__(self, path: str, kind: str = "fil"):
+        if kind not in ("fil", "psrfits"):
+            raise ValueError("kind is 'fil' or 'psrfits'")
+        self._k = kind
+        h = C.c_void_p()
+        info = FilInfo()
+        check(getattr(lib(), f"af_{kind}_open")(path.encode(), C.byref(h), C.byref(info)),
+              f"af_{kind}_open")
+        self._h = h.value
+        self.info = info
+        self.dtype = np.dtype(_DT[info.dtype])
+
+    @property
+    def nchans(self):
+        return self.info.nchans
+
+    @property
+    def ndata(self):
+        return int(self.info.ndata)
+
+    def read_into(self, row0: int, nrows: int, ptr: int):
+        """Rows [row0, row0 + nrows) into memory at ptr (e.g. a HostBuffer)."""
+        check(getattr(lib(), f"af_{self._k}_read")(self._h, row0, nrows, C.c_void_p(ptr)),
+              f"af_{self._k}_read")
+
+    def read(self, row0: int = 0, nrows: int = None) -> np.ndarray:
+        nrows = self.ndata - row0 if nrows is None else nrows
+        out = np.empty((nrows, self.nchans), self.dtype)
+        if nrows:
+            self.read_into(row0, nrows, out.ctypes.data)
+        return out
+
+    def close(self):
+        if self._h:
+            getattr(lib(), f"af_{self._k}_close")(C.c_void_p(self._h))
+            self._h = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

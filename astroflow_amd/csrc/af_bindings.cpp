@@ -590,72 +590,28 @@ struct Filterbank {
   Filterbank() = default;
   explicit Filterbank(const std::string &path) : filename(path) { read(); }
 
-  static bool read_str(std::ifstream &f, std::string &s) {
-    int32_t n = 0;
-    if (!f.read((char *)&n, 4) || n < 0 || n > 4096) return false;
-    s.resize((size_t)n);
-    return (bool)f.read(&s[0], n);
-  }
-
+  // the library's SIGPROC reader (af_fil_open / af_fil_read): header, rows,
+  // and the flip of a descending band
   void read() {
-    std::ifstream f(filename, std::ios::binary);
-    if (!f.is_open()) throw std::runtime_error("Can not open file " + filename);
-    std::string k;
-    if (!read_str(f, k) || k != "HEADER_START")
-      throw py::value_error("not a SIGPROC filterbank file: " + filename);
-    auto rd_i = [&](int &v) { int32_t x; f.read((char *)&x, 4); v = x; };
-    auto rd_d = [&](double &v) { f.read((char *)&v, 8); };
-    for (;;) {
-      if (!read_str(f, k)) throw py::value_error("truncated filterbank header: " + filename);
-      if (k == "HEADER_END") break;
-      if (k == "nchans") rd_i(nchans);
-      else if (k == "nifs") rd_i(nifs);
-      else if (k == "nbits") rd_i(nbits);
-      else if (k == "telescope_id") rd_i(telescope_id);
-      else if (k == "machine_id") rd_i(machine_id);
-      else if (k == "data_type") rd_i(data_type);
-      else if (k == "fch1") rd_d(fch1);
-      else if (k == "foff") rd_d(foff);
-      else if (k == "tstart") rd_d(tstart);
-      else if (k == "tsamp") rd_d(tsamp);
-      else if (k == "source_name" || k == "rawdatafile") {
-        std::string v;
-        read_str(f, v);
-        if (k == "source_name") source_name = v;
-      } else if (k == "nbeams" || k == "ibeam" || k == "barycentric" || k == "pulsarcentric" ||
-                 k == "nsamples") {
-        int v;
-        rd_i(v);
-      } else if (k == "src_raj" || k == "src_dej" || k == "az_start" || k == "za_start" ||
-                 k == "refdm" || k == "period") {
-        double v;
-        rd_d(v);
-      } else {
-        throw py::value_error("unsupported filterbank header keyword '" + k + "'");
-      }
-      if (!f) throw py::value_error("truncated filterbank header: " + filename);
+    af_fil_reader *r = nullptr;
+    af_fil_info info;
+    check(af_fil_open(filename.c_str(), &r, &info), "af_fil_open");
+    nchans = info.nchans;
+    nifs = 1;
+    nbits = info.nbits;
+    fch1 = info.fch1;
+    foff = info.foff;
+    tstart = info.tstart;
+    tsamp = info.tsamp;
+    ndata = (long)info.ndata;
+    raw = std::make_shared<std::vector<uint8_t>>((size_t)ndata * nchans * (size_t)(nbits / 8));
+    af_status st;
+    {
+      py::gil_scoped_release nogil;
+      st = af_fil_read(r, 0, (size_t)ndata, raw->data());
     }
-    if (nchans <= 0 || (nbits != 8 && nbits != 16 && nbits != 32))
-      throw py::value_error("unsupported filterbank layout (nchans/nbits)");
-    if (nifs != 1) throw py::value_error("only nifs == 1 is supported");
-    const std::streampos start = f.tellg();
-    f.seekg(0, std::ios::end);
-    const size_t bytes = (size_t)(f.tellg() - start);
-    const size_t row = (size_t)nchans * nifs * (size_t)(nbits / 8);
-    ndata = (long)(bytes / row);
-    raw = std::make_shared<std::vector<uint8_t>>((size_t)ndata * row);
-    f.seekg(start);
-    f.read((char *)raw->data(), (std::streamsize)raw->size());
-    if (foff < 0) {
-      const size_t es = (size_t)(nbits / 8);
-      for (long t = 0; t < ndata; ++t) {
-        uint8_t *r = raw->data() + (size_t)t * row;
-        for (int a = 0, b = nchans - 1; a < b; ++a, --b)
-          for (size_t e = 0; e < es; ++e) std::swap(r[a * es + e], r[b * es + e]);
-      }
-      fch1 = fch1 + (nchans - 1) * foff;
-      foff = -foff;
-    }
+    af_fil_close(r);
+    check(st, "af_fil_read");
   }
 
   Header header() const {
@@ -671,6 +627,45 @@ struct Filterbank {
     case 8: return py::array_t<uint8_t>(shape, (const uint8_t *)raw->data(), cap);
     case 16: return py::array_t<uint16_t>(shape, (const uint16_t *)raw->data(), cap);
     default: return py::array_t<uint32_t>(shape, (const uint32_t *)raw->data(), cap);
+    }
+  }
+};
+
+// PSRFITS search-mode file (the reference's Python PsrFits,
+// python/astroflow/io/psrfits.py:19-90, over af_psrfits_open / read):
+// pol-averaged, ascending band, [ndata][nchans].
+struct PsrFits {
+  std::string filename;
+  af_fil_info info{};
+  std::shared_ptr<std::vector<uint8_t>> raw;
+
+  explicit PsrFits(const std::string &path) : filename(path) {
+    af_psrfits_reader *r = nullptr;
+    check(af_psrfits_open(path.c_str(), &r, &info), "af_psrfits_open");
+    const size_t es = info.dtype == AF_U8 ? 1 : info.dtype == AF_U16 ? 2 : 4;
+    raw = std::make_shared<std::vector<uint8_t>>((size_t)info.ndata * info.nchans * es);
+    af_status st;
+    {
+      py::gil_scoped_release nogil;
+      st = af_psrfits_read(r, 0, (size_t)info.ndata, raw->data());
+    }
+    af_psrfits_close(r);
+    check(st, "af_psrfits_read");
+  }
+
+  Header header() const {
+    return Header((float)info.tstart, (float)info.tsamp, (float)info.fch1, (float)info.foff,
+                  info.nbits, info.nchans, (long)info.ndata, filename);
+  }
+
+  py::array data() const {
+    const std::vector<py::ssize_t> shape{(py::ssize_t)info.ndata, (py::ssize_t)info.nchans};
+    auto keep = new std::shared_ptr<std::vector<uint8_t>>(raw);
+    py::capsule cap(keep, [](void *p) { delete static_cast<std::shared_ptr<std::vector<uint8_t>> *>(p); });
+    switch (info.dtype) {
+    case AF_U8: return py::array_t<uint8_t>(shape, (const uint8_t *)raw->data(), cap);
+    case AF_U16: return py::array_t<uint16_t>(shape, (const uint16_t *)raw->data(), cap);
+    default: return py::array_t<float>(shape, (const float *)raw->data(), cap);
     }
   }
 };
@@ -945,6 +940,21 @@ PYBIND11_MODULE(_astroflow_core, m) {
       .def_readonly("ndata", &Filterbank::ndata)
       .def_property_readonly("data", &Filterbank::data)
       .def_property_readonly("header", &Filterbank::header);
+
+  py::class_<PsrFits>(m, "PsrFits")
+      .def(py::init<const std::string &>())
+      .def_readonly("filename", &PsrFits::filename)
+      .def_property_readonly("nchans", [](const PsrFits &f) { return f.info.nchans; })
+      .def_property_readonly("nbits", [](const PsrFits &f) { return f.info.nbits; })
+      .def_property_readonly("npol", [](const PsrFits &f) { return f.info.npol; })
+      .def_property_readonly("ndata", [](const PsrFits &f) { return (long)f.info.ndata; })
+      .def_property_readonly("fch1", [](const PsrFits &f) { return f.info.fch1; })
+      .def_property_readonly("foff", [](const PsrFits &f) { return f.info.foff; })
+      .def_property_readonly("tsamp", [](const PsrFits &f) { return f.info.tsamp; })
+      .def_property_readonly("mjd", [](const PsrFits &f) { return f.info.tstart; })
+      .def_property_readonly("data", &PsrFits::data)
+      .def("get_spectrum", &PsrFits::data)
+      .def_property_readonly("header", &PsrFits::header);
 
   py::class_<SpectrumOut>(m, "Spectrum")
       .def(py::init<>())

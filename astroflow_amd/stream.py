@@ -186,3 +186,35 @@ def run_rate(srv: capi.Server, beams: List[StreamBeam], pool: RowPool, rate: flo
         "hbm_peak_GB": st["hbm_peak"] / 1e9,
         "completed": int(st["completed"] - base["completed"]),
     }
+
+
+def feed_file(srv: capi.Server, beam: int, reader: "capi.FileReader", chunk_rows: int,
+              nbuf: int = 4, poll_ms: int = 5000):
+    """Streams a file through the server: chunk k's rows are read (positioned
+    reads, no intermediate copy) into page-locked slot k mod nbuf and
+    submitted; a slot is reused only after its chunk's result came back, as
+    the server borrows submitted rows until then.  Yields the server results
+    (planes still to be released by the caller) in completion order."""
+    nchunks = reader.ndata // chunk_rows
+    ring = [capi.HostBuffer((chunk_rows, reader.nchans), reader.dtype) for _ in range(nbuf)]
+    busy = {}                  # tag -> slot
+    free = list(range(nbuf))
+    k = 0
+    try:
+        while k < nchunks or busy:
+            while k < nchunks and free:
+                slot = free.pop()
+                reader.read_into(k * chunk_rows, chunk_rows, ring[slot].ptr)
+                srv.submit(beam, ring[slot].ptr, tag=k)
+                busy[k] = slot
+                k += 1
+            rs = srv.poll(64, poll_ms)
+            if not rs:
+                raise RuntimeError("server returned no result")
+            for r in rs:
+                free.append(busy.pop(r.tag))
+                yield r
+    finally:
+        srv.drain()
+        for b in ring:
+            b.free()

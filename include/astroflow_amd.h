@@ -317,6 +317,46 @@ af_status af_batch_destroy(af_batch *batch);
 af_status af_autotune(af_plan_desc *desc, const void *d_in, void *d_out, int tune_subbands,
                       int reps, float *best_ms);
 
+/* ------------------------------------------------------------- ingest --
+ * File readers that deliver rows as the plans and the server take them:
+ * time-major, channels in ascending frequency (a descending band is flipped,
+ * and the info then describes the ascending band).  Rows can be read
+ * straight into page-locked memory (af_host_alloc), which the server then
+ * uploads asynchronously (af_server_submit). */
+typedef struct af_fil_info {
+  int nchans, nbits;
+  int dtype;               /* AF_U8 / AF_U16 / AF_U32 / AF_F32 of the rows read  */
+  int npol;                /* PSRFITS: polarisations stored (0 for SIGPROC)      */
+  int flipped;             /* the stored band descended                          */
+  uint64_t ndata;          /* rows (time samples)                                */
+  uint64_t data_offset;    /* byte offset of the first row (of the SUBINT table)*/
+  double fch1, foff;       /* MHz, ascending after the flip                      */
+  double tsamp;            /* s                                                  */
+  double tstart;           /* MJD                                                */
+} af_fil_info;
+
+/* SIGPROC filterbank (replaces Filterbank::read_header, the data read and
+ * the band flip of src/data/filterbank.cpp:319-472,793-842). */
+typedef struct af_fil_reader af_fil_reader;
+af_status af_fil_open(const char *path, af_fil_reader **reader, af_fil_info *info);
+/* Rows [row0, row0 + nrows) into dst ([nrows][nchans] of nbits/8-byte
+ * samples), with positioned reads over up to 16 threads. */
+af_status af_fil_read(af_fil_reader *reader, uint64_t row0, size_t nrows, void *dst);
+af_status af_fil_close(af_fil_reader *reader);
+
+/* PSRFITS search mode (replaces PsrFits._load_data,
+ * python/astroflow/io/psrfits.py:32-76): the SUBINT table's DATA column
+ * ([NSBLK][NPOL][NCHAN] per row; 8-bit 'B', unsigned 16-bit 'I' with TZERO
+ * 32768, or float 'E'), pol 0 or the mean of pols 0 and 1 for NPOL >= 2
+ * (floor((a + b) / 2) for integers: the reference's uint8/uint16 sum wraps,
+ * this one does not), fch1 = OBSFREQ - OBSBW/2, foff = CHAN_BW, tsamp = TBIN,
+ * tstart = STT_IMJD + (STT_SMJD + STT_OFFS) / 86400. */
+typedef struct af_psrfits_reader af_psrfits_reader;
+af_status af_psrfits_open(const char *path, af_psrfits_reader **reader, af_fil_info *info);
+/* Rows [row0, row0 + nrows) into dst ([nrows][nchans] of info.dtype). */
+af_status af_psrfits_read(af_psrfits_reader *reader, uint64_t row0, size_t nrows, void *dst);
+af_status af_psrfits_close(af_psrfits_reader *reader);
+
 /* Page-locked host memory (hipHostMalloc), for asynchronous uploads. */
 af_status af_host_alloc(void **p, size_t bytes);
 af_status af_host_free(void *p);

@@ -51,3 +51,70 @@ def write_sigproc(path, data: np.ndarray, fch1: float, foff: float, tsamp: float
     with open(path, "wb") as f:
         f.write(hdr)
         f.write(np.ascontiguousarray(data).tobytes())
+
+
+def write_psrfits(path, data: np.ndarray, obsfreq: float, obsbw: float, chan_bw: float,
+                  tbin: float, nsblk: int, imjd=60000, smjd=3600, offs=0.25, extra_cols=True):
+    """A PSRFITS search-mode file: primary HDU (OBSFREQ, OBSBW, STT_*) and a
+    SUBINT binary table whose DATA column holds [NSBLK][NPOL][NCHAN] samples
+    per row (data: [nsamp][npol][nchan], nsamp a multiple of nsblk; uint8
+    'B', uint16 as 'I' with TZERO 32768, or float32 'E'), big-endian."""
+    nsamp, npol, nchan = data.shape
+    assert nsamp % nsblk == 0
+    nrow = nsamp // nsblk
+
+    def card(k, v):
+        if isinstance(v, bool):
+            val = ("T" if v else "F").rjust(20)
+        elif isinstance(v, str):
+            val = ("'" + v.ljust(8) + "'").ljust(20)
+        elif isinstance(v, int):
+            val = str(v).rjust(20)
+        else:
+            val = repr(float(v)).upper().rjust(20)
+        c = k.ljust(8) + "= " + val
+        return c.ljust(80)[:80].encode()
+
+    def block(cards):
+        b = b"".join(cards) + b"END".ljust(80)
+        return b + b" " * (-len(b) % 2880)
+
+    prim = block([card("SIMPLE", True), card("BITPIX", 8), card("NAXIS", 0), card("EXTEND", True),
+                  card("OBSFREQ", obsfreq), card("OBSBW", obsbw), card("STT_IMJD", imjd),
+                  card("STT_SMJD", smjd), card("STT_OFFS", offs)])
+    if data.dtype == np.uint8:
+        form, raw, tz = f"{nsblk * npol * nchan}B", data, None
+        nbits = 8
+    elif data.dtype == np.uint16:
+        form = f"{nsblk * npol * nchan}I"
+        raw = (data.astype(np.int32) - 32768).astype(">i2")
+        tz, nbits = 32768, 16
+    else:
+        form, raw, tz = f"{nsblk * npol * nchan}E", data.astype(">f4"), None
+        nbits = 32
+    cols = []
+    if extra_cols:
+        cols += [("TSUBINT", "1D", 8), ("DAT_SCL", f"{nchan * npol}E", 4 * nchan * npol)]
+    cols.append(("DATA", form, raw.dtype.itemsize * nsblk * npol * nchan))
+    rowbytes = sum(c[2] for c in cols)
+    cards = [card("XTENSION", "BINTABLE"), card("BITPIX", 8), card("NAXIS", 2),
+             card("NAXIS1", rowbytes), card("NAXIS2", nrow), card("PCOUNT", 0),
+             card("GCOUNT", 1), card("TFIELDS", len(cols))]
+    for i, (name, f, _) in enumerate(cols, 1):
+        cards += [card(f"TTYPE{i}", name), card(f"TFORM{i}", f)]
+        if name == "DATA" and tz is not None:
+            cards.append(card(f"TZERO{i}", tz))
+    cards += [card("EXTNAME", "SUBINT"), card("NPOL", npol), card("POL_TYPE", "AABB"),
+              card("NCHAN", nchan), card("NSBLK", nsblk), card("NBITS", nbits),
+              card("TBIN", tbin), card("CHAN_BW", chan_bw)]
+    hdr = block(cards)
+    rows = bytearray()
+    raw = raw.reshape(nrow, nsblk * npol * nchan)
+    for r in range(nrow):
+        if extra_cols:
+            rows += np.array([tbin * nsblk], ">f8").tobytes()
+            rows += np.ones(nchan * npol, ">f4").tobytes()
+        rows += raw[r].tobytes()
+    rows += b"\0" * (-len(rows) % 2880)
+    with open(path, "wb") as f:
+        f.write(prim + hdr + bytes(rows))
