@@ -253,11 +253,16 @@ __global__ __launch_bounds__(1024) void k_seg_reduce_u8(ReduceArgs a) {
 // One 1024-thread block per 32-row item.  Lane l of wave w owns channel
 // group g = 16*w + (l >> 2) (16 channels) and row quarter q = l & 3 (rows
 // 8q..8q+7): 8 16-byte loads kept in registers.  Pass 1: IQRM window mask,
-// per-row non-zero sum/count over the zero-DM range (dot4 byte sums), wave
-// reduction, LDS atomics.  Pass 2: x = trunc(max((float)x - m, 0)) inside
-// the zero-DM range, static mask, 4x4 byte transposes with v_perm, and one
-// 8-byte store per channel (the four lanes of a group write 32 contiguous
-// bytes of that channel's row).
+// per-row non-zero sum/count over the zero-DM range (v_dot4 byte sums; the
+// count as the sum minus a masked byte SAD against 1, which skips zero
+// bytes), a reduction over the row's 16 lane groups (two DPP rotates within
+// each 16-lane row, then LDS atomics from the four rows).  Pass 2: x =
+// trunc(max((float)x - m, 0)) inside the zero-DM range, static mask, 4x4
+// byte transposes with v_perm, and one 8-byte store per channel (the four
+// lanes of a group write 32 contiguous bytes of that channel's row): a
+// buffer store whose channel offset j*Lp is a scalar operand, so a store
+// costs no address arithmetic; a wave whose 256 channels are all stored
+// takes them without per-lane tests.
 __device__ __forceinline__ uint32_t zdm4(uint32_t w, float m) {
   uint32_t o = 0;
 #pragma unroll
@@ -281,12 +286,18 @@ __device__ __forceinline__ uint32_t zsub4(uint32_t w, uint32_t kk) {
   return __builtin_bit_cast(uint32_t, e) | (__builtin_bit_cast(uint32_t, o) << 8);
 }
 
-__device__ __forceinline__ uint32_t nonzero_bytes(uint32_t w) {
-  const uint32_t t = ((w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | w;   // bit 7 of each byte: byte != 0
-  return __builtin_popcount(t & 0x80808080u);
+// byte masks of channels [lo, hi) among the 16 at cb: dword k covers cb+4k..
+__device__ __forceinline__ uint32_t range16(int cb, int lo, int hi) {
+  const int a = min(max(lo - cb, 0), 16), b = min(max(hi - cb, 0), 16);
+  return b > a ? ((1u << b) - 1u) & ~((1u << a) - 1u) : 0u;
+}
+__device__ __forceinline__ uint32_t bytes_of(uint32_t m16, int k) {
+  const uint32_t nib = (m16 >> (4 * k)) & 0xFu;
+  return ((nib * 0x00204081u) & 0x01010101u) * 0xFFu;
 }
 
-__device__ __forceinline__ void clean_u8_body(const CleanArgs &a, uint32_t bid, uint32_t) {
+template <bool O32>
+__device__ __forceinline__ void clean_u8_body_t(const CleanArgs &a, uint32_t bid) {
   __shared__ uint32_t s_sum[32], s_cnt[32], s_kk[32];
   __shared__ float s_m[32];
   if (a.run_if && *a.run_if == 0) return;
@@ -301,9 +312,11 @@ __device__ __forceinline__ void clean_u8_body(const CleanArgs &a, uint32_t bid, 
     s_sum[threadIdx.x] = 0;
     s_cnt[threadIdx.x] = 0;
   }
-  // per-thread channel masks (0xFF = keep / in range)
-  // The IQRM segment mask and the static mask are 0/1 bytes: one 16-byte
-  // load each, expanded to 0x00/0xFF lanes by a multiply (no carries).
+  // per-thread channel masks (0xFF = keep / in range).  The IQRM segment
+  // mask and the static mask are 0/1 bytes: one 16-byte load each, expanded
+  // to 0x00/0xFF lanes by a multiply (no carries).
+  const uint32_t m_in = range16(cb, 0, a.nchan);
+  const uint32_t m_z = a.zdm ? range16(cb, a.zc0, a.zc1) : 0u;
   uint32_t km[4], zr[4], st[4];
   const uint8_t *smask = a.segmask ? a.segmask + (size_t)it.seg * a.nchan : nullptr;
   uint4 sm16 = make_uint4(0, 0, 0, 0), st16 = make_uint4(0, 0, 0, 0);
@@ -311,15 +324,9 @@ __device__ __forceinline__ void clean_u8_body(const CleanArgs &a, uint32_t bid, 
   if (a.stat_mask) st16 = load_row16(a.stat_mask, cb, a.nchan);
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    uint32_t inside = 0, zin = 0;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int c = cb + 4 * k + b;
-      inside |= (c < a.nchan ? 0xFFu : 0u) << (8 * b);
-      zin |= ((a.zdm && c >= a.zc0 && c < a.zc1) ? 0xFFu : 0u) << (8 * b);
-    }
+    const uint32_t inside = bytes_of(m_in, k);
     km[k] = ~(u4get(sm16, k) * 0xFFu) & inside;
-    zr[k] = zin;
+    zr[k] = bytes_of(m_z, k);
     st[k] = ~(u4get(st16, k) * 0xFFu) & inside;
   }
   uint4 w[8];
@@ -343,24 +350,26 @@ __device__ __forceinline__ void clean_u8_body(const CleanArgs &a, uint32_t bid, 
   }
   __syncthreads();
   if (a.zdm) {
+    // (row_ror:4 and row_ror:8 within each 16-lane row)
+    auto ror4 = [](uint32_t v) { return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x124, 0xF, 0xF, false); };
+    auto ror8 = [](uint32_t v) { return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xF, 0xF, false); };
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-      uint32_t s = 0, n = 0;
+      uint32_t s = 0, sad = 0;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const uint32_t z = u4get(w[i], k) & zr[k];
         s = __builtin_amdgcn_udot4(z, 0x01010101u, s, false);
-        n += nonzero_bytes(z);
+        // sum over the non-zero bytes of |1 - z| = z - 1
+        sad = __builtin_amdgcn_msad_u8(0x01010101u, z, sad);
       }
-      // per wave: s <= 1024*255 < 2^18, n <= 1024 < 2^11
-      uint32_t pk = s | (n << 18);
-      pk += __shfl_xor(pk, 4);
-      pk += __shfl_xor(pk, 8);
-      pk += __shfl_xor(pk, 16);
-      pk += __shfl_xor(pk, 32);
-      if (lane < 4) {
-        atomicAdd(&s_sum[8 * lane + i], pk & 0x3FFFFu);
-        atomicAdd(&s_cnt[8 * lane + i], pk >> 18);
+      // per 16-lane row: s <= 16*255*4 < 2^18, n <= 64 < 2^14
+      uint32_t pk = s | ((s - sad) << 18);
+      pk += ror4(pk);
+      pk += ror8(pk);
+      if ((lane & 15) < 4) {
+        atomicAdd(&s_sum[8 * q + i], pk & 0x3FFFFu);
+        atomicAdd(&s_cnt[8 * q + i], pk >> 18);
       }
     }
     __syncthreads();
@@ -374,6 +383,7 @@ __device__ __forceinline__ void clean_u8_body(const CleanArgs &a, uint32_t bid, 
       s_kk[threadIdx.x] = (fr == 0.0f || fr >= 0x1p-16f) ? (k | (k << 16)) : 0xFFFFFFFFu;
     }
     __syncthreads();
+    const bool zall = m_z == 0xFFFFu;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       const uint32_t kk = s_kk[8 * q + i];
@@ -382,7 +392,7 @@ __device__ __forceinline__ void clean_u8_body(const CleanArgs &a, uint32_t bid, 
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           const uint32_t v = u4get(w[i], k);
-          o[k] = (zsub4(v, kk) & zr[k]) | (v & ~zr[k]);
+          o[k] = zall ? zsub4(v, kk) : (zsub4(v, kk) & zr[k]) | (v & ~zr[k]);
         }
       } else {
         const float m = s_m[8 * q + i];
@@ -400,10 +410,18 @@ __device__ __forceinline__ void clean_u8_body(const CleanArgs &a, uint32_t bid, 
       w[i] = make_uint4(w[i].x & st[0], w[i].y & st[1], w[i].z & st[2], w[i].w & st[3]);
   }
   if (cb >= a.nchan) return;
-  // transpose: for channel quad k and row half h, rows 4h..4h+3
-  uint8_t *__restrict__ xT = (uint8_t *)a.xT;
+  // channels of this lane's group that are stored: [dd_c0, dd_c1), and when
+  // the item is cut, the first it.pad of them
+  const int c_hi = a.cut_in_pad ? min(a.dd_c1, a.dd_c0 + (int)it.pad) : a.dd_c1;
+  const uint32_t m_st = range16(cb, a.dd_c0, c_hi);
   const int rbase = 8 * q;
   const bool full = rbase + 8 <= nrows;
+  // wave-uniform: every lane stores all 16 channels of all 8 rows
+  const bool wave_all = __builtin_amdgcn_ballot_w64(!(full && m_st == 0xFFFFu)) == 0;
+  uint8_t *__restrict__ xT = (uint8_t *)a.xT;
+  const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(xT, (short)0, -1, 0x00020000);
+  // byte offset of (channel cb, row it.r0 + rbase); channel cb + j adds j*Lp
+  const long long cofs = (long long)(cb - a.dd_c0) * (long long)a.Lp + it.r0 + rbase;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     uint32_t lo[4], hi[4];
@@ -423,19 +441,31 @@ __device__ __forceinline__ void clean_u8_body(const CleanArgs &a, uint32_t bid, 
     }
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      const int c = cb + 4 * k + b;
-      if (c < a.dd_c0 || c >= a.dd_c1) continue;
-      if (a.cut_in_pad && c - a.dd_c0 >= (int)it.pad) continue;   // never read at these rows
-      uint8_t *dst = xT + (size_t)(c - a.dd_c0) * a.Lp + it.r0 + rbase;
+      const int j = 4 * k + b;
+      typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+      const v2u v = {lo[b], hi[b]};
+      if (O32 && wave_all) {
+        __builtin_amdgcn_raw_buffer_store_b64(v, rsrc, (int)(uint32_t)cofs,
+                                              (int)((uint32_t)j * (uint32_t)a.Lp), 0);
+        continue;
+      }
+      if (!((m_st >> j) & 1u)) continue;
+      uint8_t *dst = xT + (size_t)(cofs + (long long)j * (long long)a.Lp);
       if (full) {
-        const uint2 v = make_uint2(lo[b], hi[b]);
-        __builtin_memcpy(dst, &v, 8);
+        const uint2 u = make_uint2(lo[b], hi[b]);
+        __builtin_memcpy(dst, &u, 8);
       } else {
         for (int r = 0; r < 8; ++r)
           if (rbase + r < nrows) dst[r] = (uint8_t)(((r < 4 ? lo[b] : hi[b]) >> (8 * (r & 3))) & 0xFFu);
       }
     }
   }
+}
+
+// O32: the channel-major buffer is below 4 GiB (buffer stores with 32-bit offsets)
+__device__ __forceinline__ void clean_u8_body(const CleanArgs &a, uint32_t bid, uint32_t) {
+  if ((uint64_t)(a.dd_c1 - a.dd_c0) * a.Lp < 0xFFFFFFFFull) clean_u8_body_t<true>(a, bid);
+  else clean_u8_body_t<false>(a, bid);
 }
 
 __global__ __launch_bounds__(1024) void k_clean_u8(CleanArgs a) {
