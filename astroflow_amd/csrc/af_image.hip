@@ -123,7 +123,8 @@ hipError_t launch_rows(const void *plane, size_t ld, size_t sps, int rows, int s
 }  // namespace
 
 af_status image_slices(const void *d_plane, int dtype, size_t rows, size_t cols, size_t ld,
-                       size_t sps, int size, uint8_t *d_images, hipStream_t st) {
+                       size_t sps, int size, uint8_t *d_images, hipStream_t st, void **keep,
+                       size_t *keep_n) {
   if (!d_plane || !d_images) return fail(AF_ERR_INVALID, "null argument");
   if (dtype != AF_U32 && dtype != AF_U64 && dtype != AF_F32)
     return fail(AF_ERR_INVALID, "plane element type must be AF_U32, AF_U64 or AF_F32");
@@ -165,10 +166,25 @@ af_status image_slices(const void *d_plane, int dtype, size_t rows, size_t cols,
   const size_t tmp_b = nb * (size_t)R * size * sizeof(float);
   const size_t img_b = nb * (size_t)size * size * sizeof(float);
   const size_t blob_o = (tmp_b + img_b + nb * sizeof(float2) + 255) & ~(size_t)255;
+  // scratch: the caller's kept buffer (grown when too small) or one of its own
   uint8_t *scratch = nullptr;
-  if (hipMalloc(&scratch, blob_o + blob.size()) != hipSuccess) {
-    (void)hipGetLastError();
-    return fail(AF_ERR_NOMEM, "image scratch allocation failed");
+  const size_t need = blob_o + blob.size();
+  if (keep && *keep && *keep_n >= need) {
+    scratch = (uint8_t *)*keep;
+  } else {
+    if (keep && *keep) {
+      (void)hipFree(*keep);
+      *keep = nullptr;
+      *keep_n = 0;
+    }
+    if (hipMalloc(&scratch, need) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(AF_ERR_NOMEM, "image scratch allocation failed");
+    }
+    if (keep) {
+      *keep = scratch;
+      *keep_n = need;
+    }
   }
   float *tmp = (float *)scratch;
   float *img = (float *)(scratch + tmp_b);
@@ -207,7 +223,7 @@ af_status image_slices(const void *d_plane, int dtype, size_t rows, size_t cols,
   // the host tables and the scratch live until the work is done
   const hipError_t es = hipStreamSynchronize(st);
   if (e == hipSuccess) e = es;
-  (void)hipFree(scratch);
+  if (!keep) (void)hipFree(scratch);
   if (e != hipSuccess) return fail(AF_ERR_RUNTIME, std::string("imaging: ") + hipGetErrorString(e));
   return AF_OK;
 }
@@ -218,5 +234,5 @@ extern "C" af_status af_image_slices(const void *d_plane, int plane_dtype, size_
                                      size_t cols, size_t ld, size_t sps, int size,
                                      uint8_t *d_images, void *stream) {
   return af::image_slices(d_plane, plane_dtype, rows, cols, ld, sps, size, d_images,
-                          (hipStream_t)stream);
+                          (hipStream_t)stream, nullptr, nullptr);
 }

@@ -338,8 +338,8 @@ struct af_plan {
   hipStream_t arena_st = nullptr;
   // af_execute_host: device buffers kept between calls (grow-only) and two
   // page-locked staging buffers with their events
-  void *hx_in = nullptr, *hx_out = nullptr, *hx_masks = nullptr, *hx_img = nullptr;
-  size_t hx_in_n = 0, hx_out_n = 0, hx_masks_n = 0, hx_img_n = 0;
+  void *hx_in = nullptr, *hx_out = nullptr, *hx_masks = nullptr, *hx_img = nullptr, *hx_imgscr = nullptr;
+  size_t hx_in_n = 0, hx_out_n = 0, hx_masks_n = 0, hx_img_n = 0, hx_imgscr_n = 0;
   void *stage[2] = {nullptr, nullptr};
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
@@ -381,9 +381,12 @@ hipError_t launch_zero(uint8_t *base, size_t pitch, size_t width, int nrows, hip
 // own cleaned data: IQRM vote from p->d_segsum into p->d_masks / d_segmask,
 // and both dedispersion stages from a channel-major cleaned buffer.
 af_status run_iqrm_vote(af_plan *p, hipStream_t st);
-// DM-time images of a device plane (af_image.hip); synchronises st
+// DM-time images of a device plane (af_image.hip); synchronises st.  With
+// keep, the scratch is *keep (grown to *keep_n when needed) and stays
+// allocated for the next call; without, it is allocated and freed here.
 af_status image_slices(const void *d_plane, int dtype, size_t rows, size_t cols, size_t ld,
-                       size_t sps, int size, uint8_t *d_images, hipStream_t st);
+                       size_t sps, int size, uint8_t *d_images, hipStream_t st, void **keep,
+                       size_t *keep_n);
 af_status run_dedisp(af_plan *p, const void *xT, size_t Lp, void *d_out, hipStream_t st, int set,
                      int k_only = -1);
 }  // namespace af

@@ -150,7 +150,7 @@ static af_status dalloc(af_plan *p, void **dst, size_t bytes) {
 static void release(af_plan *p) {
   for (void *q : p->allocs) (void)hipFree(q);
   p->allocs.clear();
-  for (void **q : {&p->hx_in, &p->hx_out, &p->hx_masks, &p->hx_img})
+  for (void **q : {&p->hx_in, &p->hx_out, &p->hx_masks, &p->hx_img, &p->hx_imgscr})
     if (*q) (void)hipFree(*q), *q = nullptr;
   for (int b = 0; b < 2; ++b) {
     if (p->stage[b]) (void)hipHostFree(p->stage[b]);
@@ -1156,7 +1156,7 @@ static af_status host_call(af_plan *p, const void *h_in, void *h_out, uint8_t *h
   if (h_images) {
     const int pt = p->d.dtype == AF_F32 ? AF_F32 : p->d.dtype == AF_U32 ? AF_U64 : AF_U32;
     AF_TRY(image_slices(p->hx_out, pt, (size_t)p->ndm, p->nout, p->nout, sps, isize,
-                        (uint8_t *)p->hx_img, st));
+                        (uint8_t *)p->hx_img, st, &p->hx_imgscr, &p->hx_imgscr_n));
     e = hipMemcpyAsync(h_images, p->hx_img, img_bytes, hipMemcpyDeviceToHost, st);
   } else {
     e = staged ? download_staged(p, h_out, p->hx_out, out_bytes, st)
