@@ -169,13 +169,19 @@ def run_rate(srv: capi.Server, beams: List[StreamBeam], pool: RowPool, rate: flo
                 lat.append(r.t_done_ms - r.t_submit_ms)
                 srv.release(r.d_plane)
     wall = time.perf_counter() - t0
+    # every chunk is back: the pulse chunks' own buffers can go
+    for e in pool.extra:
+        e.free()
+    pool.extra.clear()
     st = srv.stats()
     lat = np.array(lat)
     offered = sum(b.chunk_s for b in (beams[k] for k in order))
     return {
         "rate_x_realtime": rate if rate > 0 else None,
         "chunks": int(n_total),
+        "latency_samples": int(lat.size),
         "p50_ms": float(np.percentile(lat, 50)),
+        "p95_ms": float(np.percentile(lat, 95)),
         "p99_ms": float(np.percentile(lat, 99)),
         "max_ms": float(lat.max()),
         "throughput": work / wall,

@@ -73,8 +73,9 @@ def parse():
                          "GPU only, --steps executes")
     ap.add_argument("--served-only", action="store_true",
                     help="run the served leg alone (no plain-call line)")
-    ap.add_argument("--served-chunks", type=int, default=6,
-                    help="timed chunks per beam at each arrival rate of the served leg")
+    ap.add_argument("--served-chunks", type=int, default=48,
+                    help="timed chunks per beam at each arrival rate of the served leg "
+                         "(19 beams: 912 latency samples per rate)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a known-size kernel "
                          "for scaling hardware traffic counters)")
