@@ -333,9 +333,15 @@ __device__ __forceinline__ void clean_u8_body_t(const CleanArgs &a, uint32_t bid
   {
     const uint8_t *rowp = x + (size_t)(it.r0 + 8 * q) * a.nchan + cb;
     if (cb + 16 <= a.nchan && 8 * q + 8 <= nrows) {
-      // all eight rows present: issue the eight loads back to back
+      // all eight rows present: issue the eight loads back to back.  This is
+      // the last read of the input: streaming loads leave the caches to the
+      // channel-major columns written here, which stage 1 reads next.
+      typedef uint32_t v4u_any __attribute__((ext_vector_type(4), aligned(1)));
 #pragma unroll
-      for (int i = 0; i < 8; ++i) w[i] = ld16(rowp + (size_t)i * a.nchan);
+      for (int i = 0; i < 8; ++i) {
+        const v4u_any t = __builtin_nontemporal_load((const v4u_any *)(rowp + (size_t)i * a.nchan));
+        w[i] = make_uint4(t.x, t.y, t.z, t.w);
+      }
     } else {
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
@@ -555,7 +561,9 @@ __device__ __forceinline__ void stage2_u16v_body(const Stage2Args &a, uint32_t l
     uint32_t *o = (uint32_t *)a.out + (size_t)m * a.ostride + t0 + (size_t)lane;
 #pragma unroll
     for (int i = 0; i < P; ++i)
-      if (t0 + (size_t)lane + 64 * i < a.nout) o[64 * i] = acc[i];
+      // the plane is not read again on the device: streaming stores keep it
+      // from displacing the next chunk's input in the caches
+      if (t0 + (size_t)lane + 64 * i < a.nout) __builtin_nontemporal_store(acc[i], o + 64 * i);
   }
 }
 
@@ -616,7 +624,7 @@ __device__ __forceinline__ void stage2_u16c_body(const Stage2Args &a, uint32_t l
     uint32_t *o = (uint32_t *)a.out + (size_t)m * a.ostride + t0 + (size_t)lane;
 #pragma unroll
     for (int i = 0; i < P; ++i)
-      if (t0 + (size_t)lane + 64 * i < a.nout) o[64 * i] = acc[d][i];
+      if (t0 + (size_t)lane + 64 * i < a.nout) __builtin_nontemporal_store(acc[d][i], o + 64 * i);
   }
 }
 
