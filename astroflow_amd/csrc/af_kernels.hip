@@ -784,7 +784,18 @@ __device__ __forceinline__ void dd_u8_q_body(const DdArgs &a, uint32_t lid, uint
   } else {
     uint32_t *o = (uint32_t *)a.out + orow + t;
     if (t + 16 <= a.nvalid) {
-      __builtin_memcpy(o, &tot[0], 64);
+      if (a.ngrp == 1) {
+        // the direct method's output is the plane, not read again on the
+        // device: streaming stores
+        typedef uint32_t v4u_any __attribute__((ext_vector_type(4), aligned(4)));
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const v4u_any v = {tot[4 * k], tot[4 * k + 1], tot[4 * k + 2], tot[4 * k + 3]};
+          __builtin_nontemporal_store(v, (v4u_any *)(o + 4 * k));
+        }
+      } else {
+        __builtin_memcpy(o, &tot[0], 64);
+      }
     } else {
       for (int i = 0; i < 16; ++i)
         if (t + i < a.nvalid) o[i] = tot[i];
