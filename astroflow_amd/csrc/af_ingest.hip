@@ -255,13 +255,17 @@ bool read_hdu(int fd, long long at, long long file_end, Hdu &h) {
   h.num("BITPIX", bitpix);
   h.num("PCOUNT", pc);
   h.num("GCOUNT", gc);
-  long long n = naxis > 0 ? 1 : 0;
+  if (naxis < 0 || naxis > 999 || pc < 0 || gc < 0) return false;
+  double n = naxis > 0 ? 1 : 0;
   for (int i = 1; i <= (int)naxis; ++i) {
     double v = 0;
     h.num("NAXIS" + std::to_string(i), v);
-    n *= (long long)v;
+    if (v < 0) return false;
+    n *= v;
   }
-  h.data_bytes = (long long)(std::fabs(bitpix) / 8) * (long long)gc * (n + (long long)pc);
+  const double bytes = std::fabs(bitpix) / 8 * gc * (n + pc);
+  if (!(bytes >= 0) || bytes > 9.0e18) return false;
+  h.data_bytes = (long long)bytes;
   return true;
 }
 
@@ -299,7 +303,9 @@ af_status parse_psrfits(int fd, af_psrfits_reader &r) {
       found = true;
       break;
     }
-    at = h.data_off + (h.data_bytes + 2879) / 2880 * 2880;
+    const long long next = h.data_off + (h.data_bytes + 2879) / 2880 * 2880;
+    if (next <= at) break;
+    at = next;
   }
   if (!found) return fail(AF_ERR_INVALID, "no SUBINT binary table (not PSRFITS search mode)");
   double nchan = 0, npol = 0, nsblk = 0, nbits = 0, tbin = 0, chan_bw = 0, naxis1 = 0, naxis2 = 0,
@@ -344,6 +350,8 @@ af_status parse_psrfits(int fd, af_psrfits_reader &r) {
   if (r.kind == 2) return fail(AF_ERR_INVALID, "signed 16-bit PSRFITS samples are not supported");
   if (rep != (long long)nsblk * (long long)npol * (long long)nchan)
     return fail(AF_ERR_INVALID, "DATA column size is not NSBLK x NPOL x NCHAN");
+  if (off + rep * r.es > (long long)naxis1)
+    return fail(AF_ERR_INVALID, "DATA column extends past the SUBINT row");
   double obsfreq = 0, obsbw = 0, imjd = 0, smjd = 0, offs = 0;
   h0.num("OBSFREQ", obsfreq);
   h0.num("OBSBW", obsbw);

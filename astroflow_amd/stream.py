@@ -200,7 +200,8 @@ def feed_file(srv: capi.Server, beam: int, reader: "capi.FileReader", chunk_rows
     reads, no intermediate copy) into page-locked slot k mod nbuf and
     submitted; a slot is reused only after its chunk's result came back, as
     the server borrows submitted rows until then.  Yields the server results
-    (planes still to be released by the caller) in completion order."""
+    (planes still to be released by the caller) in completion order; results
+    of other beams polled meanwhile are yielded too."""
     nchunks = reader.ndata // chunk_rows
     ring = [capi.HostBuffer((chunk_rows, reader.nchans), reader.dtype) for _ in range(nbuf)]
     busy = {}                  # tag -> slot
@@ -218,7 +219,8 @@ def feed_file(srv: capi.Server, beam: int, reader: "capi.FileReader", chunk_rows
             if not rs:
                 raise RuntimeError("server returned no result")
             for r in rs:
-                free.append(busy.pop(r.tag))
+                if r.beam == beam and r.tag in busy:   # results of other beams pass through
+                    free.append(busy.pop(r.tag))
                 yield r
     finally:
         srv.drain()

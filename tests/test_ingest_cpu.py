@@ -108,3 +108,16 @@ def test_psrfits_errors(tmp_path):
     q.write_bytes(b"not fits at all" * 300)
     with pytest.raises(ValueError):
         core.PsrFits(str(q))
+
+
+def test_psrfits_inconsistent_table_is_rejected(tmp_path):
+    data = np.zeros((64, 1, 8), np.uint8)
+    p = tmp_path / "t.sf"
+    write_psrfits(str(p), data, 1400.0, 8.0, 1.0, 1e-4, 32, extra_cols=False)
+    raw = bytearray(p.read_bytes())
+    # NAXIS1 (the row size) smaller than the DATA column
+    i = raw.index(b"NAXIS1  =")
+    raw[i:i + 80] = b"NAXIS1  =                  100".ljust(80)
+    p.write_bytes(bytes(raw))
+    with pytest.raises(ValueError, match="past the SUBINT row"):
+        core.PsrFits(str(p))
