@@ -55,7 +55,7 @@ class PlanInfo(C.Structure):
                 ("out_elem_bytes", C.c_int), ("nwin", C.c_int), ("nsb", C.c_int),
                 ("ndm_nom", C.c_int), ("max_delay", C.c_int), ("scratch_bytes", C.c_size_t),
                 ("adds", C.c_double), ("inter_bytes", C.c_size_t),
-                ("clean_passes", C.c_int)]
+                ("clean_passes", C.c_int), ("stage1_form", C.c_int)]
 
 
 class DeviceProps(C.Structure):

@@ -915,6 +915,7 @@ af_status af_plan_query(const af_plan *p, af_plan_info *info) {
   info->adds = p->adds;
   info->inter_bytes = p->nsb > 0 ? (size_t)p->ndm_nom * p->nsb * p->L1 * p->inter_elem : 0;
   info->clean_passes = p->clean_tiled ? p->nsuper : 1;
+  info->stage1_form = p->dd_lds_rows ? 16 : p->dd_q ? 8 : p->dd_fq ? 32 : 0;
   return AF_OK;
 }
 

@@ -113,6 +113,10 @@ typedef struct af_plan_info {
   size_t inter_bytes;      /* subband intermediate written by stage 1 per call  */
   int clean_passes;        /* 1: one clean pass over the whole buffer; >1: one per
                               super-tile (see clean_budget)                    */
+  int stage1_form;         /* dedispersion kernel of the first stage: 8 = u8, 8
+                              DM rows in step loading through the L1; 16 = u8,
+                              16 rows sharing an LDS window; 32 = float32 rows
+                              in step; 0 = the generic kernel                  */
 } af_plan_info;
 
 /* Per-stage device time, averaged over the n executes run since profiling
