@@ -40,6 +40,9 @@ constexpr int kDdTile = 64 * kDdLanesOut;     // outputs per wave tile
 constexpr int kDdTileQ = 63 * kDdLanesOut;
 constexpr int kDdRowsPerBlock = 4;            // DM rows (waves) per workgroup
 constexpr int kDdRowsQ = 8;                   // DM rows (waves) per workgroup, aligned-load u8 kernel
+// LDS of the u8 stage-1 kernel when a channel group spans several 256-channel
+// blocks: 16 running totals per thread
+constexpr size_t kDdTotLds = (size_t)16 * 64 * kDdRowsQ * 4;
 constexpr int kStatRows = 1024;               // rows per statistics item
 constexpr int kCleanRows = 64;                // rows per cleaning item
 constexpr size_t kInterBudget = size_t(1) << 30;   // subband intermediate per super-tile

@@ -750,20 +750,28 @@ __device__ __forceinline__ void dd_u8_q_body(const DdArgs &a, uint32_t lid, uint
       tot[4 * k + 3] = H[k] >> 16;
     }
   } else {
+    // several blocks (the direct method over more than 256 channels): the
+    // running totals wait in LDS ([16][threads], launched with kDdTotLds
+    // bytes), which keeps them out of the registers the channel loop needs
+    extern __shared__ uint32_t dd_tot[];
+    uint32_t *tl = dd_tot + threadIdx.x;
+    constexpr int NT = 64 * kDdRowsQ;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) tot[i] = 0;
+    for (int i = 0; i < 16; ++i) tl[i * NT] = 0;
     for (int b0 = ca; b0 < cb; b0 += blk_w) {
       uint32_t S[4] = {0, 0, 0, 0}, H[4] = {0, 0, 0, 0};
       dd_q_channels<O32, true>(tb, a.Lp, loff, drow, b0, min(b0 + blk_w, cb), S, H, row_ok);
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const uint32_t L = S[k] - (H[k] << 8);
-        tot[4 * k + 0] += L & 0xFFFFu;
-        tot[4 * k + 1] += H[k] & 0xFFFFu;
-        tot[4 * k + 2] += L >> 16;
-        tot[4 * k + 3] += H[k] >> 16;
+        tl[(4 * k + 0) * NT] += L & 0xFFFFu;
+        tl[(4 * k + 1) * NT] += H[k] & 0xFFFFu;
+        tl[(4 * k + 2) * NT] += L >> 16;
+        tl[(4 * k + 3) * NT] += H[k] >> 16;
       }
     }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) tot[i] = tl[i * NT];
   }
   if (lane == 63 || !row_ok) return;
   const size_t t = (size_t)tile * kDdTileQ + (size_t)lane * kDdLanesOut;
@@ -803,7 +811,10 @@ __device__ __forceinline__ void dd_u8_q_body(const DdArgs &a, uint32_t lid, uint
   }
 }
 
-__global__ __launch_bounds__(64 * kDdRowsQ) void k_dd_u8_q(DdArgs a) {
+// eight waves per SIMD (64 VGPRs): four workgroups per CU hide more of the
+// L1 miss latency than three
+__global__ __launch_bounds__(64 * kDdRowsQ) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_dd_u8_q(DdArgs a) {
   const uint32_t lid = xcd_remap(blockIdx.x, gridDim.x);
   if ((uint64_t)a.nch * a.Lp <= 0xFFFFFFFFull) dd_u8_q_body<true>(a, lid, gridDim.x);
   else dd_u8_q_body<false>(a, lid, gridDim.x);
@@ -1369,8 +1380,8 @@ AF_BATCHED(kb_segmask, SegmaskArgs, 256, (segmask_body(a, lid, nb)))
 AF_BATCHED(kb_clean_u16, CleanArgs, 256, (clean_body<uint16_t>(a, lid, nb)))
 AF_BATCHED(kb_clean_u32, CleanArgs, 256, (clean_body<uint32_t>(a, lid, nb)))
 AF_BATCHED(kb_clean_f32, CleanArgs, 256, (clean_body<float>(a, lid, nb)))
-#define AF_BATCHED_REF(NAME, ARGS, BOUNDS, CALL)                                       \
-  __global__ __launch_bounds__(BOUNDS) void NAME(                                         \
+#define AF_BATCHED_REF(NAME, ARGS, BOUNDS, ATTR, CALL)                                 \
+  __global__ __launch_bounds__(BOUNDS) ATTR void NAME(                                    \
       const ARGS *__restrict__ tab, const uint32_t *__restrict__ start, int n) {         \
     const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);                                \
     const int i = batch_find(start, n, g);                                              \
@@ -1380,10 +1391,10 @@ AF_BATCHED(kb_clean_f32, CleanArgs, 256, (clean_body<float>(a, lid, nb)))
     CALL;                                                                               \
   }
 
-AF_BATCHED_REF(kb_dd_u8q, DdArgs, 64 * kDdRowsQ,
+AF_BATCHED_REF(kb_dd_u8q, DdArgs, 64 * kDdRowsQ, __attribute__((amdgpu_waves_per_eu(8, 8))),
                 ((uint64_t)a.nch * a.Lp <= 0xFFFFFFFFull ? dd_u8_q_body<true>(a, lid, nb)
                                                          : dd_u8_q_body<false>(a, lid, nb)))
-AF_BATCHED_REF(kb_dd_f32q, DdArgs, 64 * kDdRowsQ,
+AF_BATCHED_REF(kb_dd_f32q, DdArgs, 64 * kDdRowsQ, ,
                 ((uint64_t)a.nch * a.Lp * 4 <= 0xFFFFFFFFull ? dd_f32_q_body<true>(a, lid, nb)
                                                              : dd_f32_q_body<false>(a, lid, nb)))
 AF_BATCHED(kb_dd_u8, DdArgs, 256, (dd_u8_body(a, lid, nb)))
@@ -1499,6 +1510,12 @@ hipError_t launch_clean(int dtype, const CleanArgs &a, hipStream_t st) {
   return emit(AF_KIND4(KB_CLEAN_, dtype), 6, (size_t)a.nitems, 256, 0, a, st);
 }
 
+// dynamic LDS of k_dd_u8_q: the running totals of a channel group wider than
+// one 256-channel block (only the direct method has one)
+static size_t dd_u8q_lds(const DdArgs &a) {
+  return (a.ngrp == 1 && a.nch > 256) ? kDdTotLds : 0;
+}
+
 hipError_t launch_dedisperse(int dtype, const DdArgs &a, hipStream_t st) {
   const int nrowblk = (a.nrows + kDdRowsPerBlock - 1) / kDdRowsPerBlock;
   const size_t nblk = (size_t)nrowblk * a.ntiles * a.ngrp;
@@ -1512,7 +1529,7 @@ hipError_t launch_dedisperse(int dtype, const DdArgs &a, hipStream_t st) {
     }
     if (a.u8q && (a.out16 || a.ngrp == 1))
       return emit(KB_DD_U8Q, 7, (size_t)((a.nrows + kDdRowsQ - 1) / kDdRowsQ) * a.ntiles * a.ngrp,
-                  64 * kDdRowsQ, 0, a, st);
+                  64 * kDdRowsQ, dd_u8q_lds(a), a, st);
     if (a.out16 || a.ngrp == 1) return emit(KB_DD_U8, 7, nblk, 256, 0, a, st);
     return emit(KB_DD_G_U8, 7, nblk, 256, 0, a, st);
   case AF_U16: return emit(KB_DD_G_U16, 7, nblk, 256, 0, a, st);
