@@ -574,6 +574,38 @@ SpectrumOut spectrum_at_dm(py::array_t<T, py::array::c_style> data, Header h, fl
   return r;
 }
 
+// Rows read from a file, in page-locked memory from af_host_alloc when a
+// device is present and the allocation succeeds (a host call then sends
+// them, or any row range of them, to the device without staging), else in
+// ordinary memory.
+struct HostRows {
+  uint8_t *ptr = nullptr;
+  size_t n = 0;
+  bool pinned = false;
+  std::vector<uint8_t> heap;
+
+  explicit HostRows(size_t bytes) : n(bytes) {
+    int ndev = 0;
+    void *p = nullptr;
+    if (bytes >= ((size_t)1 << 20) && af_device_count(&ndev) == AF_OK && ndev > 0 &&
+        af_host_alloc(&p, bytes) == AF_OK) {
+      ptr = (uint8_t *)p;
+      pinned = true;
+    } else {
+      heap.resize(bytes);
+      ptr = heap.data();
+    }
+  }
+  ~HostRows() {
+    if (pinned) af_host_free(ptr);
+  }
+  HostRows(const HostRows &) = delete;
+  HostRows &operator=(const HostRows &) = delete;
+  uint8_t *data() { return ptr; }
+  const uint8_t *data() const { return ptr; }
+  size_t size() const { return n; }
+};
+
 // ---------------------------------------------------- filterbank ingest --
 // SIGPROC filterbank: a keyword header between HEADER_START and HEADER_END
 // (each keyword an int32 length + text, followed by its value), then
@@ -585,7 +617,7 @@ struct Filterbank {
   double fch1 = 0, foff = 0, tstart = 0, tsamp = 0;
   long ndata = 0;
   std::string source_name;
-  std::shared_ptr<std::vector<uint8_t>> raw;   // samples as stored (after the flip)
+  std::shared_ptr<HostRows> raw;   // samples as stored (after the flip)
 
   Filterbank() = default;
   explicit Filterbank(const std::string &path) : filename(path) { read(); }
@@ -604,7 +636,7 @@ struct Filterbank {
     tstart = info.tstart;
     tsamp = info.tsamp;
     ndata = (long)info.ndata;
-    raw = std::make_shared<std::vector<uint8_t>>((size_t)ndata * nchans * (size_t)(nbits / 8));
+    raw = std::make_shared<HostRows>((size_t)ndata * nchans * (size_t)(nbits / 8));
     af_status st;
     {
       py::gil_scoped_release nogil;
@@ -621,8 +653,8 @@ struct Filterbank {
 
   py::array data() const {
     const std::vector<py::ssize_t> shape{ndata, nifs, nchans};
-    auto keep = new std::shared_ptr<std::vector<uint8_t>>(raw);
-    py::capsule cap(keep, [](void *p) { delete static_cast<std::shared_ptr<std::vector<uint8_t>> *>(p); });
+    auto keep = new std::shared_ptr<HostRows>(raw);
+    py::capsule cap(keep, [](void *p) { delete static_cast<std::shared_ptr<HostRows> *>(p); });
     switch (nbits) {
     case 8: return py::array_t<uint8_t>(shape, (const uint8_t *)raw->data(), cap);
     case 16: return py::array_t<uint16_t>(shape, (const uint16_t *)raw->data(), cap);
@@ -637,13 +669,13 @@ struct Filterbank {
 struct PsrFits {
   std::string filename;
   af_fil_info info{};
-  std::shared_ptr<std::vector<uint8_t>> raw;
+  std::shared_ptr<HostRows> raw;
 
   explicit PsrFits(const std::string &path) : filename(path) {
     af_psrfits_reader *r = nullptr;
     check(af_psrfits_open(path.c_str(), &r, &info), "af_psrfits_open");
     const size_t es = info.dtype == AF_U8 ? 1 : info.dtype == AF_U16 ? 2 : 4;
-    raw = std::make_shared<std::vector<uint8_t>>((size_t)info.ndata * info.nchans * es);
+    raw = std::make_shared<HostRows>((size_t)info.ndata * info.nchans * es);
     af_status st;
     {
       py::gil_scoped_release nogil;
@@ -660,8 +692,8 @@ struct PsrFits {
 
   py::array data() const {
     const std::vector<py::ssize_t> shape{(py::ssize_t)info.ndata, (py::ssize_t)info.nchans};
-    auto keep = new std::shared_ptr<std::vector<uint8_t>>(raw);
-    py::capsule cap(keep, [](void *p) { delete static_cast<std::shared_ptr<std::vector<uint8_t>> *>(p); });
+    auto keep = new std::shared_ptr<HostRows>(raw);
+    py::capsule cap(keep, [](void *p) { delete static_cast<std::shared_ptr<HostRows> *>(p); });
     switch (info.dtype) {
     case AF_U8: return py::array_t<uint8_t>(shape, (const uint8_t *)raw->data(), cap);
     case AF_U16: return py::array_t<uint16_t>(shape, (const uint16_t *)raw->data(), cap);

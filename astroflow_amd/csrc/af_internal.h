@@ -323,6 +323,7 @@ struct af_plan {
   size_t scratch_bytes = 0;
 
   hipStream_t stream = nullptr;
+  struct CopyPool *copy_pool = nullptr;   // host-call copy workers (af_plan.hip)
   int profiling = 0;            // 0 off, 1 every stage, 2 the dedispersion stage only
   // one set of kNumEv events per profiled execute, recorded between stages
   static constexpr int kNumEv = 7;

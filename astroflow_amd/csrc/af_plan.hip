@@ -7,11 +7,132 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <condition_variable>
+#include <map>
 #include <mutex>
 #include <thread>
 #include <vector>
 
+#include <sched.h>
+
 #include "af_internal.h"
+
+namespace {
+
+// The blocks af_host_alloc handed out (base -> bytes).  Never destroyed, so
+// that a block freed while the process exits still finds them.
+std::mutex &pinned_mu() {
+  static std::mutex *m = new std::mutex;
+  return *m;
+}
+std::map<uintptr_t, size_t> &pinned_blocks() {
+  static auto *b = new std::map<uintptr_t, size_t>;
+  return *b;
+}
+void host_pinned_add(const void *p, size_t n) {
+  std::lock_guard<std::mutex> lk(pinned_mu());
+  pinned_blocks()[(uintptr_t)p] = n;
+}
+void host_pinned_remove(const void *p) {
+  std::lock_guard<std::mutex> lk(pinned_mu());
+  pinned_blocks().erase((uintptr_t)p);
+}
+// [p, p + n) lies inside one block from af_host_alloc
+bool host_pinned(const void *p, size_t n) {
+  if (!p) return false;
+  std::lock_guard<std::mutex> lk(pinned_mu());
+  const auto &b = pinned_blocks();
+  auto it = b.upper_bound((uintptr_t)p);
+  if (it == b.begin()) return false;
+  --it;
+  return (uintptr_t)p >= it->first && (uintptr_t)p + n <= it->first + it->second;
+}
+
+// Host threads this process may run on (its CPU affinity), at most 16.
+int copy_threads() {
+  cpu_set_t cs;
+  int n = 0;
+  if (sched_getaffinity(0, sizeof(cs), &cs) == 0) n = CPU_COUNT(&cs);
+  if (n <= 0) n = (int)std::max(1u, std::thread::hardware_concurrency());
+  return std::min(n, 16);
+}
+
+}  // namespace
+
+// Workers that split a memcpy between pageable user memory and page-locked
+// staging (one thread does not reach the link's rate).  They live as long
+// as the plan, so a host call starts no threads; copy() returns when every
+// slice is done.
+struct CopyPool {
+  std::vector<std::thread> th;
+  std::mutex mu;
+  std::condition_variable go, done;
+  char *dst = nullptr;
+  const char *src = nullptr;
+  size_t n = 0, piece = 0;
+  uint64_t gen = 0;
+  int pending = 0;
+  bool stop = false;
+
+  explicit CopyPool(int workers) {
+    for (int i = 0; i < workers; ++i) th.emplace_back([this, i] { loop(i + 1); });
+  }
+  ~CopyPool() {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stop = true;
+    }
+    go.notify_all();
+    for (auto &t : th) t.join();
+  }
+  void loop(int slice) {
+    uint64_t seen = 0;
+    for (;;) {
+      char *d;
+      const char *s;
+      size_t nn, pc;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        go.wait(lk, [&] { return stop || gen != seen; });
+        if (stop) return;
+        seen = gen;
+        d = dst;
+        s = src;
+        nn = n;
+        pc = piece;
+      }
+      const size_t a = (size_t)slice * pc;
+      if (a < nn) std::memcpy(d + a, s + a, std::min(pc, nn - a));
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        if (--pending == 0) done.notify_one();
+      }
+    }
+  }
+  // slice 0 on the calling thread, slice i on worker i
+  void copy(void *d, const void *s, size_t nn) {
+    if (th.empty() || nn < ((size_t)1 << 22)) {
+      std::memcpy(d, s, nn);
+      return;
+    }
+    const size_t nt = th.size() + 1;
+    const size_t pc = ((nn + nt - 1) / nt + 4095) & ~(size_t)4095;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      dst = (char *)d;
+      src = (const char *)s;
+      n = nn;
+      piece = pc;
+      pending = (int)th.size();
+      ++gen;
+    }
+    go.notify_all();
+    std::memcpy(d, s, std::min(pc, nn));
+    std::unique_lock<std::mutex> lk(mu);
+    done.wait(lk, [&] { return pending == 0; });
+  }
+};
+
 
 namespace af {
 
@@ -160,6 +281,8 @@ static void release(af_plan *p) {
   }
   for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
   p->ev.clear();
+  delete p->copy_pool;
+  p->copy_pool = nullptr;
   if (p->stream) (void)hipStreamDestroy(p->stream);
   p->stream = nullptr;
 }
@@ -1026,28 +1149,6 @@ af_status scratch_of(int dev, uint8_t **ptr) {
 
 constexpr size_t kStageBytes = (size_t)32 << 20;   // per page-locked staging buffer
 
-// memcpy split over up to 8 host threads (pageable user memory <-> page-
-// locked staging: one thread does not reach the link's rate)
-void par_copy(void *dst, const void *src, size_t n) {
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const size_t nt = std::min<size_t>({8, hw, std::max<size_t>(1, n >> 22)});
-  if (nt <= 1) {
-    std::memcpy(dst, src, n);
-    return;
-  }
-  const size_t piece = (n / nt + 4095) & ~(size_t)4095;
-  std::vector<std::thread> th;
-  th.reserve(nt - 1);
-  for (size_t t = 1; t < nt; ++t) {
-    const size_t a = t * piece;
-    if (a >= n) break;
-    const size_t m = std::min(piece, n - a);
-    th.emplace_back([=] { std::memcpy((char *)dst + a, (const char *)src + a, m); });
-  }
-  std::memcpy(dst, src, std::min(piece, n));
-  for (auto &x : th) x.join();
-}
-
 af_status grow(void **buf, size_t *have, size_t need) {
   if (*have >= need && *buf) return AF_OK;
   if (*buf) (void)hipFree(*buf);
@@ -1061,8 +1162,10 @@ af_status grow(void **buf, size_t *have, size_t need) {
   return AF_OK;
 }
 
-// page-locked staging for af_execute_host; false when it cannot be had
+// page-locked staging for af_execute_host (and the plan's copy workers);
+// false when it cannot be had
 bool staging(af_plan *p) {
+  if (!p->copy_pool) p->copy_pool = new CopyPool(copy_threads() - 1);
   for (int b = 0; b < 2; ++b) {
     if (!p->stage[b] && hipHostMalloc(&p->stage[b], kStageBytes, hipHostMallocDefault) != hipSuccess) {
       p->stage[b] = nullptr;
@@ -1085,7 +1188,7 @@ hipError_t upload_staged(af_plan *p, void *d, const void *h, size_t n, hipStream
     const int b = (int)(k & 1);
     hipError_t e = hipEventSynchronize(p->stage_ev[b]);   // its previous DMA is done
     if (e != hipSuccess) return e;
-    par_copy(p->stage[b], (const char *)h + off, m);
+    p->copy_pool->copy(p->stage[b], (const char *)h + off, m);
     e = hipMemcpyAsync((char *)d + off, p->stage[b], m, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipEventRecord(p->stage_ev[b], st);
     if (e != hipSuccess) return e;
@@ -1108,7 +1211,7 @@ hipError_t download_staged(af_plan *p, void *h, const void *d, size_t n, hipStre
     if (prev_b >= 0) {
       const hipError_t e = hipEventSynchronize(p->stage_ev[prev_b]);
       if (e != hipSuccess) return e;
-      par_copy((char *)h + prev_off, p->stage[prev_b], prev_n);
+      p->copy_pool->copy((char *)h + prev_off, p->stage[prev_b], prev_n);
     }
     prev_b = m ? b : -1;
     prev_off = off;
@@ -1142,9 +1245,14 @@ static af_status host_call(af_plan *p, const void *h_in, void *h_out, uint8_t *h
     AF_TRY(grow(&p->hx_img, &p->hx_img_n, img_bytes));
   }
   hipStream_t st = p->stream;
-  const bool staged = staging(p);
-  hipError_t e = staged ? upload_staged(p, p->hx_in, h_in, in_bytes, st)
-                        : hipMemcpyAsync(p->hx_in, h_in, in_bytes, hipMemcpyHostToDevice, st);
+  // rows in page-locked memory from af_host_alloc go straight to the device;
+  // pageable ones through the staging buffers
+  const bool in_pinned = host_pinned(h_in, in_bytes);
+  const bool out_pinned = !h_images && host_pinned(h_out, out_bytes);
+  const bool staged = !(in_pinned && (out_pinned || h_images)) && staging(p);
+  hipError_t e = staged && !in_pinned
+                     ? upload_staged(p, p->hx_in, h_in, in_bytes, st)
+                     : hipMemcpyAsync(p->hx_in, h_in, in_bytes, hipMemcpyHostToDevice, st);
   if (e != hipSuccess) {
     (void)hipStreamSynchronize(st);
     return fail(AF_ERR_RUNTIME, std::string("H2D: ") + hipGetErrorString(e));
@@ -1160,8 +1268,9 @@ static af_status host_call(af_plan *p, const void *h_in, void *h_out, uint8_t *h
                         (uint8_t *)p->hx_img, st, &p->hx_imgscr, &p->hx_imgscr_n));
     e = hipMemcpyAsync(h_images, p->hx_img, img_bytes, hipMemcpyDeviceToHost, st);
   } else {
-    e = staged ? download_staged(p, h_out, p->hx_out, out_bytes, st)
-               : hipMemcpyAsync(h_out, p->hx_out, out_bytes, hipMemcpyDeviceToHost, st);
+    e = staged && !out_pinned
+            ? download_staged(p, h_out, p->hx_out, out_bytes, st)
+            : hipMemcpyAsync(h_out, p->hx_out, out_bytes, hipMemcpyDeviceToHost, st);
   }
   if (e == hipSuccess && dm) e = hipMemcpyAsync(h_masks, dm, m_bytes, hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -1298,11 +1407,15 @@ af_status af_host_alloc(void **p, size_t bytes) {
   if (!p) return fail(AF_ERR_INVALID, "null argument");
   hipError_t e = hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault);
   if (e != hipSuccess) return fail(AF_ERR_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+  host_pinned_add(*p, bytes ? bytes : 1);
   return AF_OK;
 }
 
 af_status af_host_free(void *p) {
-  if (p) AF_HIP(hipHostFree(p));
+  if (p) {
+    host_pinned_remove(p);
+    AF_HIP(hipHostFree(p));
+  }
   return AF_OK;
 }
 

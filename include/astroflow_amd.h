@@ -218,8 +218,10 @@ af_status af_execute_noalloc(const af_plan_desc *desc, const void *d_in, void *d
 
 /* Same on host buffers: upload, execute, download, synchronise.  The plan
  * keeps device buffers for the input and the plane from its first host call
- * on (grown when needed) and moves the data through two 32 MB page-locked
- * staging buffers; all are freed by af_plan_destroy. */
+ * on (grown when needed).  Pageable data moves through two 32 MB page-locked
+ * staging buffers, copied by the plan's own host threads; a buffer that lies
+ * wholly inside one block from af_host_alloc is transferred directly.  All
+ * are freed by af_plan_destroy. */
 af_status af_execute_host(af_plan *plan, const void *h_in, void *h_out,
                           uint8_t *h_masks);
 
@@ -361,7 +363,9 @@ af_status af_psrfits_open(const char *path, af_psrfits_reader **reader, af_fil_i
 af_status af_psrfits_read(af_psrfits_reader *reader, uint64_t row0, size_t nrows, void *dst);
 af_status af_psrfits_close(af_psrfits_reader *reader);
 
-/* Page-locked host memory (hipHostMalloc), for asynchronous uploads. */
+/* Page-locked host memory (hipHostMalloc), for asynchronous uploads.  The
+ * library remembers each block until af_host_free, so that host calls on
+ * any range inside it skip the staging copy. */
 af_status af_host_alloc(void **p, size_t bytes);
 af_status af_host_free(void *p);
 
