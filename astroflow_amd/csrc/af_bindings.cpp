@@ -22,6 +22,8 @@
 #include <string>
 #include <vector>
 
+#include <unistd.h>
+
 #include "af_cpu.h"
 #include "af_image.h"
 #include "af_snr.h"
@@ -575,20 +577,25 @@ SpectrumOut spectrum_at_dm(py::array_t<T, py::array::c_style> data, Header h, fl
 }
 
 // Rows read from a file, in page-locked memory from af_host_alloc when a
-// device is present and the allocation succeeds (a host call then sends
-// them, or any row range of them, to the device without staging), else in
-// ordinary memory.
+// device is present, the rows take at most a quarter of the machine's
+// memory and the allocation succeeds (a host call then sends them, or any
+// row range of them, to the device without staging), else in ordinary
+// memory.
 struct HostRows {
   uint8_t *ptr = nullptr;
   size_t n = 0;
   bool pinned = false;
   std::vector<uint8_t> heap;
 
+  static size_t pin_limit() {
+    const long pages = sysconf(_SC_PHYS_PAGES), psz = sysconf(_SC_PAGE_SIZE);
+    return pages > 0 && psz > 0 ? (size_t)pages * (size_t)psz / 4 : 0;
+  }
   explicit HostRows(size_t bytes) : n(bytes) {
     int ndev = 0;
     void *p = nullptr;
-    if (bytes >= ((size_t)1 << 20) && af_device_count(&ndev) == AF_OK && ndev > 0 &&
-        af_host_alloc(&p, bytes) == AF_OK) {
+    if (bytes >= ((size_t)1 << 20) && bytes <= pin_limit() && af_device_count(&ndev) == AF_OK &&
+        ndev > 0 && af_host_alloc(&p, bytes) == AF_OK) {
       ptr = (uint8_t *)p;
       pinned = true;
     } else {
