@@ -74,16 +74,26 @@ struct CopyPool {
   int pending = 0;
   bool stop = false;
 
+  // throws (std::system_error) when a thread cannot be started, after
+  // stopping the ones that were
   explicit CopyPool(int workers) {
-    for (int i = 0; i < workers; ++i) th.emplace_back([this, i] { loop(i + 1); });
+    try {
+      for (int i = 0; i < workers; ++i) th.emplace_back([this, i] { loop(i + 1); });
+    } catch (...) {
+      shutdown();
+      throw;
+    }
   }
-  ~CopyPool() {
+  ~CopyPool() { shutdown(); }
+  void shutdown() {
     {
       std::lock_guard<std::mutex> lk(mu);
       stop = true;
     }
     go.notify_all();
-    for (auto &t : th) t.join();
+    for (auto &t : th)
+      if (t.joinable()) t.join();
+    th.clear();
   }
   void loop(int slice) {
     uint64_t seen = 0;
@@ -1165,7 +1175,14 @@ af_status grow(void **buf, size_t *have, size_t need) {
 // page-locked staging for af_execute_host (and the plan's copy workers);
 // false when it cannot be had
 bool staging(af_plan *p) {
-  if (!p->copy_pool) p->copy_pool = new CopyPool(copy_threads() - 1);
+  if (!p->copy_pool) {
+    try {
+      p->copy_pool = new CopyPool(copy_threads() - 1);
+    } catch (...) {
+      p->copy_pool = nullptr;   // no threads: the runtime's own pageable copies
+      return false;
+    }
+  }
   for (int b = 0; b < 2; ++b) {
     if (!p->stage[b] && hipHostMalloc(&p->stage[b], kStageBytes, hipHostMallocDefault) != hipSuccess) {
       p->stage[b] = nullptr;
