@@ -150,6 +150,7 @@ SIGNATURES = {
     "af_batch_create": (_I, [C.POINTER(_P), _I, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
                              C.POINTER(_P)]),
     "af_batch_execute": (_I, [_P, _I, _P]),
+    "af_batch_execute_streams": (_I, [_P, _I, _I, _P]),
     "af_batch_info": (_I, [_P, C.POINTER(_I), C.POINTER(_I)]),
     "af_batch_destroy": (_I, [_P]),
     "af_autotune": (_I, [C.POINTER(PlanDesc), _P, _P, _I, _I, C.POINTER(C.c_float)]),
@@ -538,9 +539,16 @@ class Batch:
         self._h = h
         self.plans = list(plans)
 
-    def execute(self, use_graph: bool = False, stream: int = 0):
-        check(lib().af_batch_execute(self._h, int(use_graph),
-                                     C.c_void_p(stream) if stream else None), "af_batch_execute")
+    def execute(self, use_graph: bool = False, stream: int = 0, streams: int = 0):
+        """streams == 0: merged launches (af_batch_execute); streams > 0: each
+        plan's own launches, the plans spread over that many streams
+        (af_batch_execute_streams)."""
+        st = C.c_void_p(stream) if stream else None
+        if streams:
+            check(lib().af_batch_execute_streams(self._h, int(streams), int(use_graph), st),
+                  "af_batch_execute_streams")
+        else:
+            check(lib().af_batch_execute(self._h, int(use_graph), st), "af_batch_execute")
 
     def info(self):
         a, b = C.c_int(), C.c_int()

@@ -11,6 +11,11 @@
 // clean, dedispersion stage 1 and 2, mask copies), so every plan sees the
 // same dependencies as in its own execute; results are identical.  The
 // launch sequence can be captured once into a hipGraph and replayed.
+//
+// The other way to run a batch keeps each plan's launches in its own order
+// and spreads the plans over a few streams (af_batch_execute_streams):
+// different chunks' stages then run side by side on the device, the row
+// passes (HBM-bound) of one beside the dedispersion (L1-bound) of another.
 #include <algorithm>
 #include <map>
 #include <mutex>
@@ -39,11 +44,20 @@ struct af_batch {
   int device = 0;
   std::vector<af_plan *> plans;
   std::vector<af::Merged> launches;
+  std::vector<std::vector<af::LaunchDesc>> own;   // each plan's launches, in its order
   int recorded = 0;               // launches the plans would make one by one
   uint8_t *d_tables = nullptr;
   hipStream_t cap = nullptr;      // capture stream
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // per-plan streams: the side streams, fork / join events, and the graph
+  // of the fork/join captured for `side_graph_n` streams
+  std::vector<hipStream_t> side;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+  hipGraph_t side_graph = nullptr;
+  hipGraphExec_t side_exec = nullptr;
+  int side_graph_n = 0;
 };
 
 namespace af {
@@ -63,8 +77,27 @@ af_status run_launches(af_batch *b, hipStream_t st) {
 void release(af_batch *b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->side_exec) (void)hipGraphExecDestroy(b->side_exec);
+  if (b->side_graph) (void)hipGraphDestroy(b->side_graph);
   if (b->cap) (void)hipStreamDestroy(b->cap);
+  for (hipStream_t s : b->side) (void)hipStreamDestroy(s);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  for (hipEvent_t e : b->join) (void)hipEventDestroy(e);
   if (b->d_tables) (void)hipFree(b->d_tables);
+}
+
+// The plans' own launch sequences on n side streams (plan i on stream
+// i mod n), forked from `st` and joined back to it.
+af_status run_side(af_batch *b, int n, hipStream_t st) {
+  AF_HIP(hipEventRecord(b->fork, st));
+  for (int k = 0; k < n; ++k) AF_HIP(hipStreamWaitEvent(b->side[k], b->fork, 0));
+  for (size_t i = 0; i < b->own.size(); ++i)
+    for (const LaunchDesc &d : b->own[i]) AF_HIP(launch_desc(d, b->side[i % (size_t)n]));
+  for (int k = 0; k < n; ++k) {
+    AF_HIP(hipEventRecord(b->join[k], b->side[k]));
+    AF_HIP(hipStreamWaitEvent(st, b->join[k], 0));
+  }
+  return AF_OK;
 }
 
 }  // namespace
@@ -76,6 +109,9 @@ namespace af {
 // workgroups from which one launch alone keeps every CU busy (256 CUs x a
 // few resident workgroups): such launches are not merged
 constexpr uint32_t kBatchFillGrid = 2048;
+// side streams of af_batch_execute_streams (the device runs 4 hardware
+// queues per process by default; more streams share them)
+constexpr int kBatchMaxStreams = 8;
 }  // namespace af
 
 extern "C" {
@@ -225,6 +261,7 @@ af_status af_batch_create(af_plan *const *plans, int nplans, const void *const *
         m.single = d;
         b->launches.push_back(m);
       }
+  b->own = std::move(recs);
   if (!host.empty()) {
     if (hipMalloc(&b->d_tables, host.size()) != hipSuccess) {
       release(b);
@@ -261,6 +298,48 @@ af_status af_batch_execute(af_batch *b, int use_graph, void *stream) {
     AF_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   }
   AF_HIP(hipGraphLaunch(b->exec, st));
+  return AF_OK;
+}
+
+af_status af_batch_execute_streams(af_batch *b, int nstreams, int use_graph, void *stream) {
+  if (!b) return fail(AF_ERR_INVALID, "null argument");
+  if (nstreams < 1 || nstreams > kBatchMaxStreams)
+    return fail(AF_ERR_INVALID, "nstreams must be 1.." + std::to_string(kBatchMaxStreams));
+  AF_HIP(hipSetDevice(b->device));
+  hipStream_t st = (hipStream_t)stream;
+  const int n = std::min<int>(nstreams, (int)b->own.size());
+  while ((int)b->side.size() < n) {
+    hipStream_t s = nullptr;
+    AF_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    b->side.push_back(s);
+    hipEvent_t e = nullptr;
+    AF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    b->join.push_back(e);
+  }
+  if (!b->fork) AF_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+  if (!use_graph) return run_side(b, n, st);
+  if (b->side_exec && b->side_graph_n != n) {
+    (void)hipGraphExecDestroy(b->side_exec);
+    (void)hipGraphDestroy(b->side_graph);
+    b->side_exec = nullptr;
+    b->side_graph = nullptr;
+  }
+  if (!b->side_exec) {
+    if (!b->cap) AF_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+    AF_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+    af_status s = run_side(b, n, b->cap);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->cap, &g);
+    if (s != AF_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return s;
+    }
+    if (e != hipSuccess) return fail(AF_ERR_RUNTIME, std::string("graph capture: ") + hipGetErrorString(e));
+    b->side_graph = g;
+    AF_HIP(hipGraphInstantiate(&b->side_exec, b->side_graph, nullptr, nullptr, 0));
+    b->side_graph_n = n;
+  }
+  AF_HIP(hipGraphLaunch(b->side_exec, st));
   return AF_OK;
 }
 

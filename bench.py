@@ -288,6 +288,9 @@ def c3_spec(c):
         use_zero_dm=True, zdm_c0=0, zdm_c1=nchan - 1)
 
 
+BATCH_STREAMS = 4   # side streams of the streams mode (the device's hardware queues per process)
+
+
 def batch_leg(threads, budget_gb=40.0, reps=5):
     """The 48 seeded beam-chunks (uneven nchan, nsamp, DM range, dtype, ds),
     each with IQRM + zero-DM and its own seeded input (synth.c3_chunk_data,
@@ -300,13 +303,16 @@ def batch_leg(threads, budget_gb=40.0, reps=5):
                    (stage-1 form and stage-2 tile; the plane is unchanged);
       merged       the tuned plans as one af_batch: one launch per stage and
                    kernel for the whole group;
-      graph        that batch replayed from its hipGraph.
+      graph        that batch replayed from its hipGraph;
+      streams      the same batch with each plan's own launches, the plans
+                   spread over 4 streams (af_batch_execute_streams), so that
+                   different chunks' stages overlap; replayed from its graph.
     device_ms is the sum of the chunks' per-stage HIP-event times (the GPU's
     busy time); wall time above it is what merging launches can remove."""
     from astroflow_amd import capi, synth
     chunks = synth.c3_chunks(seed=4)
     budget = budget_gb * 1e9
-    modes = ("plain", "plain_tuned", "merged", "graph")
+    modes = ("plain", "plain_tuned", "merged", "graph", "streams")
     tot = {m: 0.0 for m in modes}
     spread = {m: [] for m in modes}
     device_s = 0.0
@@ -381,6 +387,8 @@ def batch_leg(threads, budget_gb=40.0, reps=5):
         spread["merged"] += ts
         med["graph"], ts = timed(lambda: batch.execute(use_graph=True))
         spread["graph"] += ts
+        med["streams"], ts = timed(lambda: batch.execute(use_graph=True, streams=BATCH_STREAMS))
+        spread["streams"] += ts
         batch.close()
         # GPU busy time of the tuned plans, from their stage events
         for p, di, do in zip(tplans, ins, outs):
@@ -419,6 +427,8 @@ def batch_leg(threads, budget_gb=40.0, reps=5):
     res["speedup_merged_vs_plain"] = tot["plain"] / tot["merged"]
     res["speedup_graph_vs_plain"] = tot["plain"] / tot["graph"]
     res["speedup_graph_vs_plain_tuned"] = tot["plain_tuned"] / tot["graph"]
+    res["speedup_streams_vs_plain"] = tot["plain"] / tot["streams"]
+    res["streams"] = BATCH_STREAMS
     # the most any launch merging could give: plain-tuned wall time over GPU
     # busy time
     res["ceiling_launch_merging"] = tot["plain_tuned"] / device_s if device_s > 0 else None
@@ -517,7 +527,8 @@ def small_batch_leg(threads, nchunk=64, reps=5):
     inf = batch.info()
     res = {}
     for name, fn in (("plain", plain), ("merged", lambda: batch.execute(use_graph=False)),
-                     ("graph", lambda: batch.execute(use_graph=True))):
+                     ("graph", lambda: batch.execute(use_graph=True)),
+                     ("streams", lambda: batch.execute(use_graph=True, streams=BATCH_STREAMS))):
         fn()
         capi.sync()
         t0 = time.perf_counter()
@@ -538,7 +549,8 @@ def small_batch_leg(threads, nchunk=64, reps=5):
             "plain_rate": work / (res["plain_ms"] / 1e3),
             "graph_rate": work / (res["graph_ms"] / 1e3),
             "speedup_merged_vs_plain": res["plain_ms"] / res["merged_ms"],
-            "speedup_graph_vs_plain": res["plain_ms"] / res["graph_ms"]}
+            "speedup_graph_vs_plain": res["plain_ms"] / res["graph_ms"],
+            "speedup_streams_vs_plain": res["plain_ms"] / res["streams_ms"]}
 
 
 def log(msg):

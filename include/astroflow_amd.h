@@ -300,6 +300,16 @@ af_status af_batch_create(af_plan *const *plans, int nplans, const void *const *
 /* use_graph != 0: the launch sequence is captured into a hipGraph on the
  * first call and replayed after.  Asynchronous on the stream. */
 af_status af_batch_execute(af_batch *batch, int use_graph, void *stream);
+/* The batch's plans, each with its own launches in its own order (nothing
+ * merged), spread over nstreams (1..8) streams of the batch: plan i on
+ * stream i mod nstreams, forked from `stream` and joined back to it, so
+ * that different chunks' stages run side by side on the device (the row
+ * passes of one beside the dedispersion of another).  use_graph != 0: the
+ * whole fork/join is captured into a hipGraph on the first call (again
+ * when nstreams changes) and replayed after.  Asynchronous on the stream;
+ * results equal the plans' own executes.  Replaces, like af_batch_execute,
+ * the reference's one chunk per call (python/astroflow/search.py:365,464). */
+af_status af_batch_execute_streams(af_batch *batch, int nstreams, int use_graph, void *stream);
 /* Launches per execute of the batch, and of the plans run one by one. */
 af_status af_batch_info(const af_batch *batch, int *launches, int *recorded);
 af_status af_batch_destroy(af_batch *batch);

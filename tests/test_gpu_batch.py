@@ -1,8 +1,10 @@
 """Batched execution: plans of different nchan / nsamp / DM range / dtype /
 method run with one merged launch per stage for their small launches (a
-launch that fills the GPU runs alone), replayed from a hipGraph; every
-output equals the plan's own execute bit for bit; the autotuner picks only
-result-invariant settings unless asked to tune the subband width."""
+launch that fills the GPU runs alone), replayed from a hipGraph, or each
+with its own launches spread over a few streams (also replayed from a
+graph); every output equals the plan's own execute bit for bit; the
+autotuner picks only result-invariant settings unless asked to tune the
+subband width."""
 import numpy as np
 import pytest
 
@@ -86,11 +88,20 @@ def test_batched_outputs_equal_single_executes(gpu_lib):
     info = batch.info()
     say(f"launches {info}")
     assert info["launches"] < info["recorded"]
-    for mode in ("merged", "graph", "graph", "graph"):
+    # merged launches, their graph replayed three times; then the plans'
+    # own launches on 4 streams, that fork/join graph replayed three times,
+    # recaptured for 2 streams, and on 1 stream
+    modes = [("merged", False, 0), ("graph", True, 0), ("graph", True, 0), ("graph", True, 0),
+             ("streams 4", False, 4), ("streams 4 graph", True, 4), ("streams 4 graph", True, 4),
+             ("streams 4 graph", True, 4), ("streams 2 graph", True, 2), ("streams 1", False, 1)]
+    for mode, use_graph, nst in modes:
         for b in d_out:
             capi.check(capi.lib().af_memset(capi.C.c_void_p(b.ptr), 0, b.nbytes, None))
+        for b, pl in zip(d_m, plans):
+            if pl.nwin:
+                capi.check(capi.lib().af_memset(capi.C.c_void_p(b.ptr), 0, b.nbytes, None))
         capi.sync()
-        batch.execute(use_graph=(mode == "graph"))
+        batch.execute(use_graph=use_graph, streams=nst)
         capi.sync()
         for i, (p, (r, rm)) in enumerate(zip(plans, ref)):
             got = d_out[i].download(np.empty(p.out_shape, p.out_dtype))
@@ -100,6 +111,8 @@ def test_batched_outputs_equal_single_executes(gpu_lib):
                 assert np.array_equal(gm, rm), (mode, i)
         # host work between replays
         _ = sum(range(10000))
+    with pytest.raises(ValueError, match="nstreams"):
+        batch.execute(streams=9)
     batch.close()
     for p in plans:
         p.close()
