@@ -48,13 +48,17 @@ bool host_pinned(const void *p, size_t n) {
   return (uintptr_t)p >= it->first && (uintptr_t)p + n <= it->first + it->second;
 }
 
-// Host threads this process may run on (its CPU affinity), at most 16.
+// Host threads of a staged copy: the CPUs this process may run on (its
+// affinity), at most 8.  Eight already move pageable memory as fast as the
+// link takes it; sixteen measured no faster, and on a machine whose CPU
+// quota is 16 they, with the runtime's own threads, sometimes got the
+// process throttled (a 15 ms stall in a 20 ms call).
 int copy_threads() {
   cpu_set_t cs;
   int n = 0;
   if (sched_getaffinity(0, sizeof(cs), &cs) == 0) n = CPU_COUNT(&cs);
   if (n <= 0) n = (int)std::max(1u, std::thread::hardware_concurrency());
-  return std::min(n, 16);
+  return std::min(n, 8);
 }
 
 }  // namespace
