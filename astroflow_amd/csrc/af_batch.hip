@@ -53,6 +53,8 @@ struct af_batch {
   // per-plan streams: the side streams, fork / join events, and the graph
   // of the fork/join captured for `side_graph_n` streams
   std::vector<hipStream_t> side;
+  std::vector<int> side_of;       // plan -> side stream, for side_of_n streams
+  int side_of_n = 0;
   hipEvent_t fork = nullptr;
   std::vector<hipEvent_t> join;
   hipGraph_t side_graph = nullptr;
@@ -86,13 +88,34 @@ void release(af_batch *b) {
   if (b->d_tables) (void)hipFree(b->d_tables);
 }
 
-// The plans' own launch sequences on n side streams (plan i on stream
-// i mod n), forked from `st` and joined back to it.
+// Plans onto n side streams, largest first onto the least loaded (a plan's
+// load: the threads its launches start).
+void assign_side(af_batch *b, int n) {
+  const size_t np = b->own.size();
+  std::vector<double> load(np, 0.0);
+  for (size_t i = 0; i < np; ++i)
+    for (const LaunchDesc &d : b->own[i]) load[i] += (double)d.nblk * (double)d.threads;
+  std::vector<size_t> order(np);
+  for (size_t i = 0; i < np; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return load[x] > load[y]; });
+  std::vector<double> acc((size_t)n, 0.0);
+  b->side_of.assign(np, 0);
+  for (size_t i : order) {
+    const int k = (int)(std::min_element(acc.begin(), acc.end()) - acc.begin());
+    b->side_of[i] = k;
+    acc[(size_t)k] += load[i];
+  }
+  b->side_of_n = n;
+}
+
+// The plans' own launch sequences on n side streams (assign_side), forked
+// from `st` and joined back to it.
 af_status run_side(af_batch *b, int n, hipStream_t st) {
+  if (b->side_of_n != n) assign_side(b, n);
   AF_HIP(hipEventRecord(b->fork, st));
   for (int k = 0; k < n; ++k) AF_HIP(hipStreamWaitEvent(b->side[k], b->fork, 0));
   for (size_t i = 0; i < b->own.size(); ++i)
-    for (const LaunchDesc &d : b->own[i]) AF_HIP(launch_desc(d, b->side[i % (size_t)n]));
+    for (const LaunchDesc &d : b->own[i]) AF_HIP(launch_desc(d, b->side[(size_t)b->side_of[i]]));
   for (int k = 0; k < n; ++k) {
     AF_HIP(hipEventRecord(b->join[k], b->side[k]));
     AF_HIP(hipStreamWaitEvent(st, b->join[k], 0));

@@ -301,8 +301,9 @@ af_status af_batch_create(af_plan *const *plans, int nplans, const void *const *
  * first call and replayed after.  Asynchronous on the stream. */
 af_status af_batch_execute(af_batch *batch, int use_graph, void *stream);
 /* The batch's plans, each with its own launches in its own order (nothing
- * merged), spread over nstreams (1..8) streams of the batch: plan i on
- * stream i mod nstreams, forked from `stream` and joined back to it, so
+ * merged), spread over nstreams (1..8) streams of the batch (largest plans
+ * first, each onto the least loaded stream), forked from `stream` and
+ * joined back to it, so
  * that different chunks' stages run side by side on the device (the row
  * passes of one beside the dedispersion of another).  use_graph != 0: the
  * whole fork/join is captured into a hipGraph on the first call (again
