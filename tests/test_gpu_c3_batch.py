@@ -2,7 +2,9 @@
 synth.c3_chunks(seed=4) (nchan 1024/2048/4096, nsamp log-uniform 2^14-2^20,
 u8 or float32, DM 0-500/2000/5000 step 1 or 0.5, ds 1/2/4, IQRM + zero-DM,
 32-channel subbands), each with its own seeded input (noise plus narrowband
-RFI), run through af_batch with hipGraph replay, four chunks per batch.
+RFI), run through af_batch with hipGraph replay, four chunks per batch, both
+as merged launches and on per-chunk streams (af_batch_execute_streams); the
+two give the same rows and masks.
 
 Per chunk:
   - every IQRM mask equals the oracle's, bit for bit;
@@ -70,6 +72,19 @@ def test_c3_batch_parity(gpu_lib, batch):
     batch_h = capi.Batch([it["plan"] for it in items], [it["di"].ptr for it in items],
                          [it["do"].ptr for it in items], [it["dm"].ptr for it in items])
     say(f"batch {batch}: {batch_h.info()}")
+    # per-chunk streams first (its graph captured and replayed): its sampled
+    # rows and masks must equal the merged replay's, checked below
+    batch_h.execute(use_graph=True, streams=4)
+    capi.sync()
+    by_streams = []
+    for it in items:
+        rows = spread_rows(it["plan"].info.ndm)
+        by_streams.append((fetch_rows(it["do"].ptr, rows, it["plan"].info.nout, it["plan"].out_dtype),
+                           it["dm"].download(np.empty((it["plan"].nwin, it["spec"].nchan), np.uint8))))
+    for it in items:
+        capi.check(capi.lib().af_memset(capi.C.c_void_p(it["do"].ptr), 0xA5, it["do"].nbytes, None))
+        capi.check(capi.lib().af_memset(capi.C.c_void_p(it["dm"].ptr), 0xA5, it["dm"].nbytes, None))
+    capi.sync()
     batch_h.execute(use_graph=True)          # capture + first replay
     capi.sync()
     for it in items:                          # overwrite: the replay below must rewrite all
@@ -78,13 +93,15 @@ def test_c3_batch_parity(gpu_lib, batch):
     capi.sync()
     batch_h.execute(use_graph=True)
     capi.sync()
-    for it in items:
+    for it, (s_rows, s_mask) in zip(items, by_streams):
         plan, spec, x = it["plan"], it["spec"], it["x"]
         nout, ndm = plan.info.nout, plan.info.ndm
         odt = plan.out_dtype
         rows = spread_rows(ndm)
         got = fetch_rows(it["do"].ptr, rows, nout, odt)
         gmask = it["dm"].download(np.empty((plan.nwin, spec.nchan), np.uint8))
+        assert np.array_equal(got, s_rows) and np.array_equal(gmask, s_mask), \
+            f"chunk {it['i']}: per-chunk streams differ from the merged replay"
         say(f"chunk {it['i']}: oracle ({len(rows)} rows)")
         s = oracle_spec(spec)
         ref, rmask, cleaned = O.run_pipeline_rows(x, s, rows, O.IQRM_GPU)
