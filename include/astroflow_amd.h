@@ -307,7 +307,10 @@ af_status af_batch_execute(af_batch *batch, int use_graph, void *stream);
  * that different chunks' stages run side by side on the device (the row
  * passes of one beside the dedispersion of another).  use_graph != 0: the
  * whole fork/join is captured into a hipGraph on the first call (again
- * when nstreams changes) and replayed after.  Asynchronous on the stream;
+ * when nstreams changes) and replayed after.  Keep one nstreams per batch:
+ * after a recapture the runtime's streams for the new graph's branches
+ * can share hardware queues (measured 4.6 ms instead of 3.8 ms on 64
+ * small chunks); the form without a graph is as fast on those chunks.  Asynchronous on the stream;
  * results equal the plans' own executes.  Replaces, like af_batch_execute,
  * the reference's one chunk per call (python/astroflow/search.py:365,464). */
 af_status af_batch_execute_streams(af_batch *batch, int nstreams, int use_graph, void *stream);
